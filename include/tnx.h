@@ -1,0 +1,181 @@
+/*
+ * tnx — C-ABI of the MI355X finite-DMRG hot path (libtnx.so).
+ *
+ * The reference (tanlin2013/tnpy) is pure Python: its hot path is a set of
+ * Python call sites into quimb / primme / LAPACK. Each entry point below
+ * replaces one of those call sites; the comment on each names the
+ * reference interface it stands in for (file:line in tnpy).
+ *
+ * Conventions
+ *  - All tensor arguments are device pointers (HBM), caller-owned, dense,
+ *    row-major, FP64 unless stated. The library never frees caller memory.
+ *  - MPS site tensor A[l][p][r]  ('lpr', tnpy/matrix_product_state.py:45).
+ *    Edge sites use a bond of size 1.
+ *  - MPO tensor W[wl][wr][k][b]  (quimb 'lrud', k = ket, b = bra,
+ *    tnpy/operators.py:97-98), edge tensors padded to a bond of size 1.
+ *  - Environments L[l_ket][w][l_bra], R[r_ket][w][r_bra]
+ *    (tnpy/matrix_product_state.py:386-407,427-437).
+ *  - Scratch memory is a caller-supplied workspace; its size comes from the
+ *    matching *_workspace_bytes query.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t, NULL = the
+ *    default stream) and reentrant. Calls that return host scalars
+ *    (tnx_eigsh_heff1, tnx_syev_lowest's info) synchronise that stream.
+ *  - Status: 0 on success, negative on failure; tnx_last_error() returns a
+ *    thread-local message. No C++ exception crosses this boundary.
+ */
+#ifndef TNX_H_
+#define TNX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* tnx_stream_t; /* hipStream_t */
+
+enum {
+  TNX_OK = 0,
+  TNX_ERR_INVALID = -1,     /* bad argument (null pointer, bad dims, small workspace) */
+  TNX_ERR_HIP = -2,         /* HIP runtime error */
+  TNX_ERR_NOCONV = -3,      /* iterative solver hit its iteration limit */
+  TNX_ERR_UNSUPPORTED = -4  /* shape outside what the kernels support */
+};
+
+/* Arithmetic of a contraction. FP32 and BF16X3 read FP64 operands, convert
+ * while staging tiles in LDS, accumulate in FP32 on MFMA, and write FP64. */
+enum { TNX_F64 = 0, TNX_F32 = 1, TNX_BF16X3 = 2 };
+
+const char* tnx_last_error(void);
+int tnx_version(void);
+
+/* ---- generic batched GEMM (building block, FP64 operands) -------------
+ * C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]; A(m,k) = a_kcontig ?
+ * A[m*lda+k] : A[k*lda+m]; B(k,n) = b_ncontig ? B[k*ldb+n] : B[n*ldb+k];
+ * C(m,n) = C[m*ldc+n]. Replaces the numpy tensordot / BLAS dgemm calls that
+ * quimb issues (tnpy/matrix_product_state.py:315,336,438). */
+int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
+                 const double* A, int64_t lda, int64_t strideA, int a_kcontig,
+                 const double* B, int64_t ldb, int64_t strideB, int b_ncontig,
+                 double beta, double* C, int64_t ldc, int64_t strideC,
+                 int precision, tnx_stream_t stream);
+
+/* ---- one-site effective Hamiltonian matvec -----------------------------
+ * y[l'][b][r'] = sum L[l][wl][l'] W[wl][wr][k][b] R[r][wr][r'] x[l][k][r]
+ * Replaces Environment.one_site_matvec's closure
+ * (tnpy/matrix_product_state.py:411-440), the LinearOperator primme calls. */
+size_t tnx_heff1_workspace_bytes(int chil, int Dl, int Dr, int d, int chir);
+int tnx_heff1_matvec(const double* L, const double* W, const double* R,
+                     const double* x, double* y, int chil, int Dl, int Dr,
+                     int d, int chir, int precision, void* ws, size_t ws_bytes,
+                     tnx_stream_t stream);
+
+/* Dense H_eff, M[(l,k,r)][(l',b,r')]. Replaces
+ * Environment.one_site_full_matrix (tnpy/matrix_product_state.py:372-409). */
+int tnx_heff1_dense(const double* L, const double* W, const double* R,
+                    double* M, int chil, int Dl, int Dr, int d, int chir,
+                    tnx_stream_t stream);
+
+/* ---- environment updates ----------------------------------------------
+ * Lout[r][wr][r'] = sum L[l][wl][l'] A[l][k][r] W[wl][wr][k][b] A[l'][b][r']
+ * Replaces Environment.update_left (tnpy/matrix_product_state.py:296-315). */
+size_t tnx_env_workspace_bytes(int chil, int Dl, int Dr, int d, int chir);
+int tnx_env_left(const double* L, const double* A, const double* W,
+                 double* Lout, int chil, int Dl, int Dr, int d, int chir,
+                 int precision, void* ws, size_t ws_bytes, tnx_stream_t stream);
+/* Rout[l][wl][l'] = sum R[r][wr][r'] A[l][k][r] W[wl][wr][k][b] A[l'][b][r']
+ * Replaces Environment.update_right (tnpy/matrix_product_state.py:317-336). */
+int tnx_env_right(const double* R, const double* A, const double* W,
+                  double* Rout, int chil, int Dl, int Dr, int d, int chir,
+                  int precision, void* ws, size_t ws_bytes, tnx_stream_t stream);
+
+/* ---- small dense symmetric eigensolver (lowest pair) -------------------
+ * M: n x n symmetric (device), n <= 256. Writes the unit eigenvector of
+ * the smallest eigenvalue to evec (device, n) and the eigenvalue to
+ * *eval_host. Replaces linalg.eigh -> np.linalg.eigh (tnpy/linalg.py:42-61)
+ * as used by FiniteDMRG.one_site_solver below exact_solver_dim
+ * (tnpy/finite_dmrg.py:108-110). Synchronises the stream. */
+size_t tnx_syev_lowest_workspace_bytes(int n);
+int tnx_syev_lowest(const double* M, int n, double* evec, double* eval_host,
+                    void* ws, size_t ws_bytes, tnx_stream_t stream);
+
+/* ---- Krylov lowest eigenpair of the one-site H_eff ---------------------
+ * Thick-restart Lanczos with full CGS2 reorthogonalisation. Converged when
+ * the residual norm ||H y - theta y|| <= tol * ||H||_est (||H||_est = largest
+ * |Ritz value| seen, PRIMME's criterion). With precision != TNX_F64 the
+ * Krylov matvecs run in that precision and the Ritz pair is corrected by
+ * FP64 residuals until the FP64 criterion holds.
+ * v0 (device, N=chil*d*chir) is the start vector; vec (device, N) receives
+ * the unit eigenvector; hvec (device, N, may be NULL) receives H*vec.
+ * Replaces linalg.eigshmv -> primme.eigsh(which='SA', k=1)
+ * (tnpy/linalg.py:64-87) as called at tnpy/finite_dmrg.py:111. */
+typedef struct {
+  double tol;         /* residual tolerance (relative to ||H||_est) */
+  int max_basis;      /* Lanczos basis size before a thick restart (>= 3) */
+  int max_matvecs;    /* iteration limit (all precisions) */
+  int precision;      /* TNX_F64 / TNX_F32 / TNX_BF16X3 for the Krylov matvecs */
+  int reserved;
+} tnx_eigsh_opts_t;
+
+typedef struct {
+  double eval;        /* Rayleigh quotient (FP64) of the returned vector */
+  double resnorm;     /* residual norm of the returned pair */
+  double anorm;       /* ||H|| estimate used by the criterion */
+  int matvecs;        /* total H_eff applications */
+  int matvecs_f64;    /* of which in FP64 */
+  int restarts;
+  int converged;      /* 1 if the criterion was met */
+} tnx_eigsh_info_t;
+
+size_t tnx_eigsh_heff1_workspace_bytes(int chil, int Dl, int Dr, int d,
+                                       int chir, int max_basis);
+int tnx_eigsh_heff1(const double* L, const double* W, const double* R,
+                    const double* v0, double* vec, double* hvec, int chil,
+                    int Dl, int Dr, int d, int chir,
+                    const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
+                    void* ws, size_t ws_bytes, tnx_stream_t stream);
+
+/* ---- gauge moves (untruncated) ----------------------------------------
+ * Rightward: A (chil*d x chir) = Q R; A <- Q (left-orthonormal);
+ *   next_out (chir x next_cols) = R * next. Replaces
+ *   MatrixProductState.split_tensor RIGHTWARD + linalg.svd
+ *   (tnpy/matrix_product_state.py:200-211, tnpy/linalg.py:9-23): the
+ *   untruncated SVD U*(sV^T) is replaced by Q*R, the same state in another
+ *   gauge of the bond.
+ * Leftward: A (chil x d*chir) = L Q; A <- Q (right-orthonormal);
+ *   prev_out (prev_rows x chil) = prev * L. Replaces the LEFTWARD branch
+ *   (tnpy/matrix_product_state.py:212-223). next/prev must not alias the
+ *   outputs. */
+size_t tnx_gauge_workspace_bytes(int chil, int d, int chir);
+int tnx_gauge_right(double* A, const double* next, double* next_out,
+                    int chil, int d, int chir, int next_cols, void* ws,
+                    size_t ws_bytes, tnx_stream_t stream);
+int tnx_gauge_left(double* A, const double* prev, double* prev_out,
+                   int chil, int d, int chir, int prev_rows, void* ws,
+                   size_t ws_bytes, tnx_stream_t stream);
+
+/* Householder QR of a row-major m x n matrix, m >= n: A <- Q (m x n,
+ * orthonormal columns), R (n x n, upper triangular, row-major).
+ * Building block of the gauge moves (numpy.linalg.qr / LAPACK dgeqrf+dorgqr,
+ * tnpy/linalg.py:26-39). */
+size_t tnx_qr_workspace_bytes(int m, int n);
+int tnx_qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes,
+           tnx_stream_t stream);
+
+/* ---- BLAS-1 (FP64, HBM bound) ------------------------------------------
+ * Perturbation psi += alpha * H psi (tnpy/finite_dmrg.py:139-141) is
+ * tnx_axpy with alpha = 1e-5. tnx_dot writes x.y to *out_host and
+ * synchronises; its reduction order is fixed (deterministic). */
+int tnx_axpy(int64_t n, double alpha, const double* x, double* y,
+             tnx_stream_t stream);
+int tnx_scal(int64_t n, double alpha, double* x, tnx_stream_t stream);
+size_t tnx_dot_workspace_bytes(int64_t n);
+int tnx_dot(int64_t n, const double* x, const double* y, double* out_host,
+            void* ws, size_t ws_bytes, tnx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNX_H_ */

@@ -1,0 +1,196 @@
+"""End-to-end FiniteDMRG on the GPU against ED, closed forms and the CPU
+oracle run from the same initial state."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dmrg_ref as o
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def quiet_logger():
+    import logging
+
+    lg = logging.getLogger("tnpy")
+    old = lg.level
+    lg.setLevel(logging.WARNING)
+    yield
+    lg.setLevel(old)
+
+
+def test_reference_xxz_n10(gpu, fixtures):
+    # tests/test_finite_dmrg.py:9-23: XXZ(n=10, 0.5), bond_dim=32, tol 1e-8
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    d = FiniteDMRG(XXZ(n=10, delta=0.5).mpo, bond_dim=2**5, seed=1)
+    e = d.run(tol=1e-8)[-1]
+    assert abs(e - fixtures["brute_force"]["xxz_n10_delta0.5_E0"]) < 1e-8
+
+
+def test_readme_api_xxz_n16(gpu, fixtures):
+    # README.md:96-101 form: FiniteDMRG(mpo=..., chi=...).update(tol=...)
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    d = FiniteDMRG(mpo=XXZ(n=16, delta=0.5).mpo, chi=32, seed=7)
+    e = d.update(tol=1e-8)[-1]
+    E0 = fixtures["brute_force"]["xxz_n16_delta0.5_E0"]
+    assert abs(e - E0) < 1e-8
+    assert d.stats["dense_sites"] > 0 and d.stats["krylov_sites"] > 0
+
+
+def test_ferromagnet_closed_form(gpu):
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    d = FiniteDMRG(XXZ(n=16, delta=1.0).mpo, bond_dim=16, seed=3)
+    e = d.run(tol=1e-10, max_sweep=20)[-1]
+    assert abs(e - (-(16 - 1) / 4)) < 1e-10
+
+
+def test_random_field_vs_ed(gpu):
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import RandomHeisenberg
+
+    m = RandomHeisenberg(n=10, h=0.5, seed=2024)
+    E0 = np.linalg.eigvalsh(o.full_hamiltonian(m.mpo.padded))[0]
+    d = FiniteDMRG(m.mpo, bond_dim=32, seed=5)
+    assert abs(d.run(tol=1e-10)[-1] - E0) < 1e-9
+
+
+def test_random_coupling_vs_ed(gpu):
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import RandomCouplingHeisenberg
+
+    m = RandomCouplingHeisenberg(n=12, seed=4)
+    H = o.full_hamiltonian(m.mpo.padded)
+    E0 = np.linalg.eigvalsh(H)[0]
+    d = FiniteDMRG(m.mpo, bond_dim=64, seed=5)
+    assert abs(d.run(tol=1e-10)[-1] - E0) < 1e-9
+
+
+def test_parity_with_oracle_same_start(gpu):
+    """Same seeded initial MPS, same tolerance: per-sweep energies and the
+    normalised Schmidt spectra of the final state agree with the oracle."""
+    from tnpy_amd import FiniteDMRG, MatrixProductState
+    from tnpy_amd.model import XXZ
+
+    n, chi = 14, 24
+    mpo = o.xxz_mpo(n, 0.5)
+    start = o.random_mps(n, chi, 2, seed=21)
+    ref = o.OracleDMRG(mpo, start)
+    e_ref = ref.run(tol=1e-12, max_sweep=6, with_variance=False)
+    d = FiniteDMRG(XXZ(n=n, delta=0.5).mpo, bond_dim=chi, mps=MatrixProductState(start), compute_variance=False)
+    e_gpu = d.run(tol=1e-12, max_sweep=6)
+    assert len(e_gpu) == len(e_ref)
+    np.testing.assert_allclose(e_gpu, e_ref, rtol=1e-10, atol=0)
+    s_ref = o.schmidt_values(ref.mps)
+    s_gpu = d.mps.schmidt_values()
+    for a, b in zip(s_gpu, s_ref):
+        np.testing.assert_allclose(np.sort(a)[::-1], b, atol=1e-7)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mixed_precision_energy(gpu, precision):
+    from tnpy_amd import FiniteDMRG, MatrixProductState
+    from tnpy_amd.model import XXZ
+
+    n, chi = 16, 32
+    start = o.random_mps(n, chi, 2, seed=8)
+    ref = FiniteDMRG(XXZ(n=n, delta=0.5).mpo, bond_dim=chi, mps=MatrixProductState(start), compute_variance=False)
+    e64 = ref.run(tol=1e-10, max_sweep=8)
+    mix = FiniteDMRG(XXZ(n=n, delta=0.5).mpo, bond_dim=chi, mps=MatrixProductState(start),
+                     precision=precision, compute_variance=False)
+    emx = mix.run(tol=1e-10, max_sweep=8)
+    assert abs(emx[-1] - e64[-1]) < 1e-10
+    assert mix.stats["matvecs"] > mix.stats["matvecs_f64"]
+
+
+def test_variance_matches_ed(gpu):
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    mpo = XXZ(n=8, delta=0.5).mpo
+    d = FiniteDMRG(mpo, bond_dim=8, seed=2)
+    d.run(tol=1e-6, max_sweep=2)
+    v = d.mps.to_dense()
+    H = o.full_hamiltonian(mpo.padded)
+    ref = v @ H @ H @ v - (v @ H @ v) ** 2
+    assert abs(d.variances[-1] - ref) < 1e-10
+    assert abs(d.measurements.expectation_value(mpo) - v @ H @ v) < 1e-11
+
+
+def test_measurements_before_run_raises(gpu):
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    d = FiniteDMRG(XXZ(n=6, delta=0.5).mpo, bond_dim=4, seed=0)
+    with pytest.raises(RuntimeError):
+        d.measurements
+
+
+def test_max_sweep_warning(gpu, caplog):
+    import logging
+
+    from tnpy_amd import FiniteDMRG
+    from tnpy_amd.model import XXZ
+
+    logging.getLogger("tnpy").setLevel(logging.INFO)
+    d = FiniteDMRG(XXZ(n=10, delta=0.5).mpo, bond_dim=8, seed=0, compute_variance=False)
+    with caplog.at_level(logging.WARNING, logger="tnpy"):
+        d.run(tol=1e-14, max_sweep=2)
+    assert any("Maximum number of sweeps" in r.message for r in caplog.records)
+
+
+def test_mps_random_and_split(gpu):
+    from tnpy_amd import Direction, MatrixProductState
+
+    mps = MatrixProductState.random(n=8, bond_dim=10, phys_dim=2, seed=3)
+    assert [mps[i].shape for i in (0, 1, 2, 7)] == [(2, 2), (2, 2, 4), (4, 2, 8), (2, 2)]
+    assert abs(mps.norm() - 1) < 1e-12
+    v0 = mps.to_dense()
+    for site in (2, 3, 4, 6):
+        mps.split_tensor(site, Direction.RIGHTWARD)
+        np.testing.assert_allclose(mps.to_dense(), v0, atol=1e-12)
+    mps.split_tensor(5, Direction.LEFTWARD)
+    np.testing.assert_allclose(mps.to_dense(), v0, atol=1e-12)
+
+
+def test_save_load_roundtrip(gpu, tmp_path):
+    from tnpy_amd import MatrixProductState
+
+    mps = MatrixProductState.random(n=12, bond_dim=8, phys_dim=2, seed=3)
+    f = tmp_path / "state.npz"
+    mps.save(str(f))
+    back = MatrixProductState.load(str(f))
+    for a, b in zip(mps.arrays, back.arrays):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(ValueError):
+        mps.save(str(tmp_path / "state.txt"))
+
+
+def test_heff_matches_full_hamiltonian_projection(gpu):
+    """H_eff(site) = P^T H P with P the embedding of the site tensor."""
+    from tnpy_amd import Environment, MatrixProductState
+    from tnpy_amd.model import RandomHeisenberg
+
+    n = 7
+    mpo = RandomHeisenberg(n=n, h=0.5, seed=9).mpo
+    arrays = o.random_mps(n, 4, 2, seed=10)
+    env = Environment(mpo, MatrixProductState(arrays), build_left=True)
+    H = o.full_hamiltonian(mpo.padded)
+    for site in range(n):
+        M = env.one_site_full_matrix(site)
+        N = M.shape[0]
+        P = np.zeros((2**n, N))
+        for c in range(N):
+            arr = [a.copy() for a in arrays]
+            arr[site] = np.eye(N)[:, c].reshape(arrays[site].shape)
+            P[:, c] = o.mps_to_dense(arr)
+        np.testing.assert_allclose(M, P.T @ H @ P, atol=1e-11)
+        op = env.one_site_matvec(site)
+        x = np.random.default_rng(site).standard_normal(N)
+        np.testing.assert_allclose(op.matvec(x).ravel(), M.T @ x, atol=1e-12)

@@ -1,0 +1,297 @@
+"""GPU kernels of libtnx against the CPU oracle, through the C-ABI.
+
+Tolerances: FP64 paths to ~1e-13 relative; FP32 / BF16x3 contractions
+within their arithmetic's rounding (stated per test); iterative solvers to
+their residual criterion.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import dmrg_ref as o
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def L():
+    from tnpy_amd import _lib
+
+    return _lib.lib()
+
+
+def ws(nbytes):
+    buf = torch.empty(int(nbytes) + 256, dtype=torch.uint8, device="cuda")
+    return buf, buf.data_ptr(), buf.numel()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def check(st, what="call"):
+    assert st == 0, f"{what}: status {st}: {L().tnx_last_error()}"
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+# relative Frobenius error bounds of one contraction in each arithmetic
+TOL = {"fp64": 1e-13, "fp32": 3e-6, "bf16x3": 1e-4}
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("ak, bn", [(1, 1), (1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize("M, N, K", [(1, 1, 1), (17, 33, 9), (130, 70, 260), (256, 384, 512)])
+def test_gemm(gpu, prec, ak, bn, M, N, K):
+    print(f"gemm {prec} ak={ak} bn={bn} {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    Cin = rng.standard_normal((M, N))
+    Ad = dev(A if ak else A.T)
+    Bd = dev(B if bn else B.T)
+    Cd = dev(Cin)
+    lda = K if ak else M
+    ldb = N if bn else K
+    check(L().tnx_gemm_f64(M, N, K, 1, 0.75, Ad.data_ptr(), lda, 0, ak, Bd.data_ptr(), ldb, 0, bn,
+                           -0.5, Cd.data_ptr(), N, 0, PREC[prec], stream()), "gemm")
+    ref = 0.75 * A @ B - 0.5 * Cin
+    assert relerr(host(Cd), ref) < TOL[prec]
+
+
+def test_gemm_batched(gpu):
+    rng = np.random.default_rng(5)
+    bt, M, N, K = 5, 40, 24, 33
+    A = rng.standard_normal((bt, M, K))
+    B = rng.standard_normal((bt, K, N))
+    out = torch.zeros(bt, M, N, dtype=torch.float64, device="cuda")
+    check(L().tnx_gemm_f64(M, N, K, bt, 1.0, dev(A).data_ptr(), K, M * K, 1, dev(B).data_ptr(), N, K * N, 1,
+                           0.0, out.data_ptr(), N, M * N, 0, stream()))
+    assert relerr(host(out), A @ B) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+def rand_env(rng, chil, Dl, Dr, d, chir):
+    Lt = rng.standard_normal((chil, Dl, chil))
+    Rt = rng.standard_normal((chir, Dr, chir))
+    W = rng.standard_normal((Dl, Dr, d, d))
+    x = rng.standard_normal((chil, d, chir))
+    return Lt, W, Rt, x
+
+
+SHAPES = [(1, 1, 5, 2, 2), (2, 5, 5, 2, 4), (8, 5, 5, 2, 16), (33, 5, 5, 2, 17), (64, 5, 5, 2, 64),
+          (16, 5, 1, 2, 1), (12, 25, 25, 2, 12), (10, 6, 6, 2, 20), (9, 4, 3, 3, 7)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_heff1_matvec(gpu, prec, shape):
+    chil, Dl, Dr, d, chir = shape
+    print(f"matvec {prec} {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    Lt, W, Rt, x = rand_env(rng, *shape)
+    y = torch.empty(chil * d * chir, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_heff1_workspace_bytes(*shape))
+    check(L().tnx_heff1_matvec(dev(Lt).data_ptr(), dev(W).data_ptr(), dev(Rt).data_ptr(), dev(x).data_ptr(),
+                               y.data_ptr(), *shape, PREC[prec], wp, wb, stream()), "matvec")
+    ref = o.heff_matvec(Lt, W, Rt, x)
+    assert relerr(host(y), ref) < 10 * TOL[prec]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_heff1_dense(gpu, shape):
+    chil, Dl, Dr, d, chir = shape
+    rng = np.random.default_rng(3 + sum(shape))
+    Lt, W, Rt, _ = rand_env(rng, *shape)
+    N = chil * d * chir
+    M = torch.empty(N, N, dtype=torch.float64, device="cuda")
+    check(L().tnx_heff1_dense(dev(Lt).data_ptr(), dev(W).data_ptr(), dev(Rt).data_ptr(), M.data_ptr(), *shape,
+                              stream()))
+    assert relerr(host(M), o.heff_dense(Lt, W, Rt)) < 1e-14
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_env_update(gpu, side, shape):
+    chil, Dl, Dr, d, chir = shape
+    print(f"env {side} {shape}", flush=True)
+    rng = np.random.default_rng(11 + sum(shape))
+    Lt, W, Rt, A = rand_env(rng, *shape)
+    _, wp, wb = ws(L().tnx_env_workspace_bytes(*shape))
+    if side == "left":
+        out = torch.empty(chir, Dr, chir, dtype=torch.float64, device="cuda")
+        check(L().tnx_env_left(dev(Lt).data_ptr(), dev(A).data_ptr(), dev(W).data_ptr(), out.data_ptr(), *shape,
+                               0, wp, wb, stream()))
+        ref = o.env_left(Lt, A, W)
+    else:
+        out = torch.empty(chil, Dl, chil, dtype=torch.float64, device="cuda")
+        check(L().tnx_env_right(dev(Rt).data_ptr(), dev(A).data_ptr(), dev(W).data_ptr(), out.data_ptr(), *shape,
+                                0, wp, wb, stream()))
+        ref = o.env_right(Rt, A, W)
+    assert relerr(host(out), ref) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 16, 64, 100, 199, 256])
+def test_syev_lowest(gpu, n):
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    M = X + X.T
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ev = C.c_double()
+    _, wp, wb = ws(L().tnx_syev_lowest_workspace_bytes(n))
+    check(L().tnx_syev_lowest(dev(M).data_ptr(), n, v.data_ptr(), C.byref(ev), wp, wb, stream()))
+    w = np.linalg.eigvalsh(M)
+    nrm = np.abs(w).max()
+    assert abs(ev.value - w[0]) <= 1e-13 * nrm
+    vv = host(v)
+    assert abs(np.linalg.norm(vv) - 1) < 1e-12
+    assert np.linalg.norm(M @ vv - ev.value * vv) <= 1e-12 * nrm
+
+
+def test_syev_degenerate(gpu):
+    # ferromagnet-like exact degeneracy of the lowest level
+    n = 40
+    Q, _ = np.linalg.qr(np.random.default_rng(2).standard_normal((n, n)))
+    w = np.concatenate([[-3.0, -3.0, -3.0], np.linspace(-1, 2, n - 3)])
+    M = (Q * w) @ Q.T
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ev = C.c_double()
+    _, wp, wb = ws(L().tnx_syev_lowest_workspace_bytes(n))
+    check(L().tnx_syev_lowest(dev(M).data_ptr(), n, v.data_ptr(), C.byref(ev), wp, wb, stream()))
+    vv = host(v)
+    assert abs(ev.value + 3.0) < 1e-13
+    assert np.linalg.norm(M @ vv + 3.0 * vv) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+def physical_env(n, chi, site, seed, model="xxz"):
+    """Environments of a real DMRG problem (oracle), for solver tests."""
+    mpo = o.xxz_mpo(n, 0.5) if model == "xxz" else o.random_heisenberg_mpo(n, 0.5, seed=seed)
+    mps = o.random_mps(n, chi, 2, seed=seed)
+    # left-canonical to the left of `site`
+    for i in range(site):
+        o.split_tensor_svd(mps, i, o.RIGHTWARD)
+    d = o.OracleDMRG(mpo, mps)
+    return d.left[site], mpo[site], d.right[site], mps[site]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("n, chi, site", [(10, 32, 4), (12, 16, 6), (14, 48, 7)])
+def test_eigsh(gpu, prec, n, chi, site):
+    from tnpy_amd import _lib
+
+    print(f"eigsh {prec} n={n} chi={chi} site={site}", flush=True)
+    Lt, W, Rt, A = physical_env(n, chi, site, seed=n + chi)
+    M = o.heff_dense(Lt, W, Rt)
+    w, V = np.linalg.eigh(M)
+    shape = (Lt.shape[0], W.shape[0], W.shape[1], W.shape[2], Rt.shape[0])
+    N = A.size
+    vec = torch.empty(N, dtype=torch.float64, device="cuda")
+    hvec = torch.empty(N, dtype=torch.float64, device="cuda")
+    opts = _lib.EigshOpts(tol=1e-10, max_basis=24, max_matvecs=3000, precision=PREC[prec])
+    info = _lib.EigshInfo()
+    _, wp, wb = ws(L().tnx_eigsh_heff1_workspace_bytes(*shape, 24))
+    check(L().tnx_eigsh_heff1(dev(Lt).data_ptr(), dev(W).data_ptr(), dev(Rt).data_ptr(), dev(A).data_ptr(),
+                              vec.data_ptr(), hvec.data_ptr(), *shape, C.byref(opts), C.byref(info), wp, wb,
+                              stream()), "eigsh")
+    nrm = np.abs(w).max()
+    assert info.converged == 1
+    assert abs(info.eval - w[0]) <= 1e-12 * nrm, (info.eval, w[0], info.matvecs)
+    v = host(vec)
+    assert abs(np.linalg.norm(v) - 1) < 1e-12
+    assert np.linalg.norm(M @ v - w[0] * v) <= 2e-10 * nrm
+    np.testing.assert_allclose(host(hvec), M @ v, atol=1e-11 * nrm)
+    if prec != "fp64":
+        assert info.matvecs_f64 >= 1
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("m, n", [(1, 1), (7, 3), (37, 23), (64, 64), (512, 256), (600, 300),
+                                  (1100, 513), (2048, 1024), (3000, 700), (5000, 300)])
+def test_qr(gpu, m, n):
+    print(f"qr {m}x{n}", flush=True)
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n))
+    Ad = dev(A)
+    R = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_qr_workspace_bytes(m, n))
+    check(L().tnx_qr(Ad.data_ptr(), m, n, R.data_ptr(), wp, wb, stream()), "qr")
+    Q = host(Ad)
+    Rh = host(R)
+    assert np.abs(np.tril(Rh, -1)).max() == 0.0
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13 * np.sqrt(n) * 10
+    assert relerr(Q @ Rh, A) < 1e-13 * 10
+
+
+def test_qr_rank_deficient(gpu):
+    # rank-3 2000x600 matrix plus exact zero columns: Q must stay orthonormal
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((2000, 3)) @ rng.standard_normal((3, 600))
+    A[:, 100:110] = 0.0
+    Ad = dev(A)
+    R = torch.empty(600, 600, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_qr_workspace_bytes(2000, 600))
+    check(L().tnx_qr(Ad.data_ptr(), 2000, 600, R.data_ptr(), wp, wb, stream()), "qr")
+    Q = host(Ad)
+    assert np.linalg.norm(Q.T @ Q - np.eye(600)) < 1e-11
+    assert relerr(Q @ host(R), A) < 1e-12
+
+
+@pytest.mark.parametrize("chil, d, chir", [(1, 2, 2), (4, 2, 8), (32, 2, 32), (16, 2, 32), (60, 2, 60), (100, 2, 50)])
+def test_gauge_moves(gpu, chil, d, chir):
+    rng = np.random.default_rng(chil * chir)
+    A = rng.standard_normal((chil, d, chir))
+    nxt = rng.standard_normal((chir, d, 3))
+    prv = rng.standard_normal((5, d, chil))
+    gp, wp, wb = ws(L().tnx_gauge_workspace_bytes(chil, d, chir))
+    if chil * d >= chir:
+        Ad, out = dev(A), torch.empty(chir, d, 3, dtype=torch.float64, device="cuda")
+        check(L().tnx_gauge_right(Ad.data_ptr(), dev(nxt).data_ptr(), out.data_ptr(), chil, d, chir, d * 3,
+                                  wp, wb, stream()), "gauge_right")
+        Q = host(Ad).reshape(chil * d, chir)
+        assert np.linalg.norm(Q.T @ Q - np.eye(chir)) < 1e-12
+        before = A.reshape(chil * d, chir) @ nxt.reshape(chir, -1)
+        after = Q @ host(out).reshape(chir, -1)
+        assert relerr(after, before) < 1e-13
+    if d * chir >= chil:
+        Ad, out = dev(A), torch.empty(5, d, chil, dtype=torch.float64, device="cuda")
+        check(L().tnx_gauge_left(Ad.data_ptr(), dev(prv).data_ptr(), out.data_ptr(), chil, d, chir, 5 * d,
+                                 wp, wb, stream()), "gauge_left")
+        Q = host(Ad).reshape(chil, d * chir)
+        assert np.linalg.norm(Q @ Q.T - np.eye(chil)) < 1e-12
+        before = prv.reshape(-1, chil) @ A.reshape(chil, -1)
+        after = host(out).reshape(-1, chil) @ Q
+        assert relerr(after, before) < 1e-13
+
+
+def test_blas1(gpu):
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(100003), rng.standard_normal(100003)
+    xd, yd = dev(x), dev(y)
+    check(L().tnx_axpy(x.size, 0.25, xd.data_ptr(), yd.data_ptr(), stream()))
+    np.testing.assert_allclose(host(yd), y + 0.25 * x, rtol=0, atol=1e-15)
+    check(L().tnx_scal(x.size, -2.0, xd.data_ptr(), stream()))
+    np.testing.assert_array_equal(host(xd), -2.0 * x)
+    out = C.c_double()
+    _, wp, wb = ws(L().tnx_dot_workspace_bytes(x.size))
+    check(L().tnx_dot(x.size, xd.data_ptr(), yd.data_ptr(), C.byref(out), wp, wb, stream()))
+    assert abs(out.value - (-2.0 * x) @ (y + 0.25 * x)) < 1e-10
+    # deterministic: same bits on repeat
+    out2 = C.c_double()
+    check(L().tnx_dot(x.size, xd.data_ptr(), yd.data_ptr(), C.byref(out2), wp, wb, stream()))
+    assert out.value == out2.value

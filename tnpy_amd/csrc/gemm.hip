@@ -1,0 +1,369 @@
+// Strided, batched GEMM on CDNA4 matrix cores, the contraction engine of
+// every tensor operation in the library (H_eff matvec stages, environment
+// updates, QR trailing updates, gauge absorption).
+//
+// One kernel template, three arithmetics:
+//   F64    : v_mfma_f64_16x16x4_f64, FP64 LDS tiles [BK][BM]
+//   F32    : v_mfma_f32_16x16x4_f32, FP32 LDS tiles [BK][BM]
+//   BF16X3 : v_mfma_f32_16x16x32_bf16 three times per tile pair,
+//            a*b ~ ahi*bhi + ahi*blo + alo*bhi with a = ahi + alo (bf16),
+//            FP32 accumulate, LDS tiles [BM][BK] so each lane reads its
+//            8 consecutive k values with one 16-byte LDS read.
+// Operands may be FP64 or FP32 in HBM (runtime, wave-uniform branch); they
+// are converted while staging the tile in LDS. Block = 256 threads as a 2x2
+// grid of 64-wide wavefronts, each wave owning a (BM/2)x(BN/2) sub-tile of
+// 16x16 MFMA accumulators. Global loads of tile t+1 are issued before the
+// MFMAs of tile t; LDS is double-buffered (one barrier per k-tile).
+#include "tnx_common.h"
+
+namespace tnx {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A; int64_t lda, sA; int ta;
+  const void* B; int64_t ldb, sB; int tb;
+  void* C; int64_t ldc, sC; int tc;
+  double alpha, beta;
+  int tiles_n;  // number of tiles along N
+  int kc;       // K extent of one split (== K when not split)
+  double* partial;  // split-K partial sums [batch][split][M][N], or null
+};
+
+__device__ __forceinline__ double ld_any(const void* p, int t, int64_t i) {
+  return t == 0 ? reinterpret_cast<const double*>(p)[i]
+                : (double)reinterpret_cast<const float*>(p)[i];
+}
+
+template <int PREC> struct PT;
+template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; using accs_t = double; static constexpr int BK = 16; };
+template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 16; };
+template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 32; };
+
+template <int PREC, int BM, int BN, bool A_K, bool B_N>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  using lds_t = typename PT<PREC>::lds_t;
+  using acc_t = typename PT<PREC>::acc_t;
+  constexpr int BK = PT<PREC>::BK;
+  constexpr int NT = 256;
+  constexpr int EA = BM * BK / NT;
+  constexpr int EB = BK * BN / NT;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  // LDS geometry. F64/F32: [BK][BM + PADM]. BF16X3: [BM][BK + PADK] for
+  // hi and lo parts (the k index is innermost so a lane's 8 values are
+  // one 16-byte read).
+  constexpr int PADM = (PREC == 0) ? 2 : 4;
+  constexpr int PADK = 8;
+  constexpr int A_ELEMS = (PREC == 2) ? 2 * BM * (BK + PADK) : BK * (BM + PADM);
+  constexpr int B_ELEMS = (PREC == 2) ? 2 * BN * (BK + PADK) : BK * (BN + PADM);
+  // 16-byte alignment: the BF16X3 path reads 8 bf16 values per lane at once
+  __shared__ __attribute__((aligned(16))) lds_t As[2][A_ELEMS];
+  __shared__ __attribute__((aligned(16))) lds_t Bs[2][B_ELEMS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM;
+  const int wn0 = (wave & 1) * WN;
+
+  const int bat = blockIdx.y;
+  const int tile = blockIdx.x;
+  const int split = blockIdx.z;
+  const int kbeg = split * g.kc;
+  const int kend = min(g.K, kbeg + g.kc);
+  const int m0 = (tile / g.tiles_n) * BM;
+  const int n0 = (tile % g.tiles_n) * BN;
+
+  const char* Ab = (const char*)g.A + (size_t)bat * g.sA * (g.ta == 0 ? 8 : 4);
+  const char* Bb = (const char*)g.B + (size_t)bat * g.sB * (g.tb == 0 ? 8 : 4);
+
+  float ra[EA], rb[EB];
+  double da[PREC == 0 ? EA : 1], db[PREC == 0 ? EB : 1];
+
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int mi, ki;
+      if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
+      const int m = m0 + mi, k = k0 + ki;
+      double v = 0.0;
+      if (m < g.M && k < kend) {
+        const int64_t off = A_K ? (int64_t)m * g.lda + k : (int64_t)k * g.lda + m;
+        v = ld_any(Ab, g.ta, off);
+      }
+      if constexpr (PREC == 0) da[e] = v; else ra[e] = (float)v;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT;
+      int ni, ki;
+      if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
+      const int n = n0 + ni, k = k0 + ki;
+      double v = 0.0;
+      if (n < g.N && k < kend) {
+        const int64_t off = B_N ? (int64_t)k * g.ldb + n : (int64_t)n * g.ldb + k;
+        v = ld_any(Bb, g.tb, off);
+      }
+      if constexpr (PREC == 0) db[e] = v; else rb[e] = (float)v;
+    }
+  };
+
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + e * NT;
+      int mi, ki;
+      if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
+      if constexpr (PREC == 0) {
+        As[buf][ki * (BM + PADM) + mi] = da[e];
+      } else if constexpr (PREC == 1) {
+        As[buf][ki * (BM + PADM) + mi] = ra[e];
+      } else {
+        const float x = ra[e];
+        const __bf16 hi = (__bf16)x;
+        const __bf16 lo = (__bf16)(x - (float)hi);
+        As[buf][mi * (BK + PADK) + ki] = hi;
+        As[buf][BM * (BK + PADK) + mi * (BK + PADK) + ki] = lo;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + e * NT;
+      int ni, ki;
+      if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
+      if constexpr (PREC == 0) {
+        Bs[buf][ki * (BN + PADM) + ni] = db[e];
+      } else if constexpr (PREC == 1) {
+        Bs[buf][ki * (BN + PADM) + ni] = rb[e];
+      } else {
+        const float x = rb[e];
+        const __bf16 hi = (__bf16)x;
+        const __bf16 lo = (__bf16)(x - (float)hi);
+        Bs[buf][ni * (BK + PADK) + ki] = hi;
+        Bs[buf][BN * (BK + PADK) + ni * (BK + PADK) + ki] = lo;
+      }
+    }
+  };
+
+  acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int nk = (kend - kbeg + BK - 1) / BK;
+  load_tiles(kbeg);
+  store_tiles(0);
+  __syncthreads();
+
+  const int lr = lane & 15;   // row (A) / column (B) inside a 16x16 tile
+  const int lk = lane >> 4;   // k group of the lane
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * BK);
+    const lds_t* as = As[buf];
+    const lds_t* bs = Bs[buf];
+    if constexpr (PREC == 0 || PREC == 1) {
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        lds_t a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = as[(kk + lk) * (BM + PADM) + wm0 + i * 16 + lr];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = bs[(kk + lk) * (BN + PADM) + wn0 + j * 16 + lr];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (PREC == 0)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+          }
+      }
+    } else {
+      bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+      constexpr int LO_A = BM * (BK + PADK);
+      constexpr int LO_B = BN * (BK + PADK);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm0 + i * 16 + lr;
+        ah[i] = *reinterpret_cast<const bf16x8*>(&as[r * (BK + PADK) + lk * 8]);
+        al[i] = *reinterpret_cast<const bf16x8*>(&as[LO_A + r * (BK + PADK) + lk * 8]);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int c = wn0 + j * 16 + lr;
+        bh[j] = *reinterpret_cast<const bf16x8*>(&bs[c * (BK + PADK) + lk * 8]);
+        bl[j] = *reinterpret_cast<const bf16x8*>(&bs[LO_B + c * (BK + PADK) + lk * 8]);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nk) store_tiles(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue: D(i, j) of a 16x16 tile sits in lane j + 16*(i/4), item i%4.
+  if (g.partial) {
+    double* P = g.partial + ((size_t)bat * gridDim.z + split) * (size_t)g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = m0 + wm0 + i * 16 + lk * 4 + v;
+          const int n = n0 + wn0 + j * 16 + lr;
+          if (m < g.M && n < g.N) P[(size_t)m * g.N + n] = (double)acc[i][j][v];
+        }
+    return;
+  }
+  char* Cb = (char*)g.C + (size_t)bat * g.sC * (g.tc == 0 ? 8 : 4);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm0 + i * 16 + lk * 4 + v;
+        const int n = n0 + wn0 + j * 16 + lr;
+        if (m < g.M && n < g.N) {
+          const int64_t off = (int64_t)m * g.ldc + n;
+          double val = g.alpha * (double)acc[i][j][v];
+          if (g.tc == 0) {
+            double* c = reinterpret_cast<double*>(Cb) + off;
+            if (g.beta != 0.0) val += g.beta * *c;
+            *c = val;
+          } else {
+            float* c = reinterpret_cast<float*>(Cb) + off;
+            if (g.beta != 0.0) val += g.beta * (double)*c;
+            *c = (float)val;
+          }
+        }
+      }
+}
+
+// C = alpha * sum_z partial[bat][z] + beta * C   (fixed summation order)
+__global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, int M, int N,
+                                     void* C, int tc, int64_t ldc, int64_t sC, double alpha,
+                                     double beta) {
+  const int bat = blockIdx.y;
+  const int64_t MN = (int64_t)M * N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < MN;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int z = 0; z < splits; ++z) acc += P[((int64_t)bat * splits + z) * MN + e];
+    const int64_t m = e / N, n = e % N;
+    const int64_t off = (int64_t)bat * sC + m * ldc + n;
+    double val = alpha * acc;
+    if (tc == 0) {
+      double* c = reinterpret_cast<double*>(C) + off;
+      if (beta != 0.0) val += beta * *c;
+      *c = val;
+    } else {
+      float* c = reinterpret_cast<float*>(C) + off;
+      if (beta != 0.0) val += beta * (double)*c;
+      *c = (float)val;
+    }
+  }
+}
+
+template <int PREC, int BM, int BN>
+static int launch_tiles(const GemmDesc& d, hipStream_t s) {
+  GemmArgs g;
+  g.M = d.M; g.N = d.N; g.K = d.K;
+  g.A = d.A; g.lda = d.lda; g.sA = d.sA; g.ta = (int)d.ta;
+  g.B = d.B; g.ldb = d.ldb; g.sB = d.sB; g.tb = (int)d.tb;
+  g.C = d.C; g.ldc = d.ldc; g.sC = d.sC; g.tc = (int)d.tc;
+  g.alpha = d.alpha; g.beta = d.beta;
+  g.tiles_n = ceil_div(d.N, BN);
+  const int tiles = ceil_div(d.M, BM) * g.tiles_n;
+  // Split K when the output tiles cannot fill the chip and K is long.
+  int splits = 1;
+  g.kc = d.K;
+  g.partial = nullptr;
+  const int64_t blocks = (int64_t)tiles * d.batch;
+  if (d.splitk_ws && blocks < 128 && d.K >= 512) {
+    constexpr int BK = PT<PREC>::BK;
+    int want = (int)std::min<int64_t>(256 / std::max<int64_t>(blocks, 1), d.K / 256);
+    want = std::max(1, std::min(want, 32));
+    const size_t need = (size_t)want * d.batch * d.M * d.N * sizeof(double);
+    if (want > 1 && need <= d.splitk_ws_bytes) {
+      splits = want;
+      g.kc = ((d.K + splits - 1) / splits + BK - 1) / BK * BK;
+      splits = ceil_div(d.K, g.kc);
+      g.partial = d.splitk_ws;
+    }
+  }
+  dim3 grid(tiles, d.batch, splits);
+  dim3 block(256);
+  if (d.a_kcontig && d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, true>), grid, block, 0, s, g);
+  else if (d.a_kcontig && !d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, false>), grid, block, 0, s, g);
+  else if (!d.a_kcontig && d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, true>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, false>), grid, block, 0, s, g);
+  TNX_LAUNCH_CHECK();
+  if (g.partial) {
+    const int64_t MN = (int64_t)d.M * d.N;
+    const int rb = (int)std::min<int64_t>((MN + 255) / 256, 2048);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb, d.batch), dim3(256), 0, s, g.partial,
+                       splits, d.M, d.N, d.C, (int)d.tc, d.ldc, d.sC, d.alpha, d.beta);
+    TNX_LAUNCH_CHECK();
+  }
+  return TNX_OK;
+}
+
+template <int PREC>
+static int launch_prec(const GemmDesc& d, hipStream_t s) {
+  const int64_t big = (int64_t)ceil_div(d.M, 128) * ceil_div(d.N, 128) * d.batch;
+  if (big >= 200) return launch_tiles<PREC, 128, 128>(d, s);
+  return launch_tiles<PREC, 64, 64>(d, s);
+}
+
+int gemm(const GemmDesc& d, hipStream_t s) {
+  TNX_CHECK_ARG(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 0, "gemm: negative size");
+  if (d.M == 0 || d.N == 0 || d.batch == 0) return TNX_OK;
+  TNX_CHECK_ARG(d.A && d.B && d.C, "gemm: null operand");
+  TNX_CHECK_ARG(d.batch <= 65535, "gemm: batch %d exceeds 65535", d.batch);
+  switch (d.prec) {
+    case Prec::F64: return launch_prec<0>(d, s);
+    case Prec::F32: return launch_prec<1>(d, s);
+    case Prec::BF16X3: return launch_prec<2>(d, s);
+  }
+  set_error("gemm: unknown precision");
+  return TNX_ERR_INVALID;
+}
+
+}  // namespace tnx
+
+extern "C" int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
+                            const double* A, int64_t lda, int64_t strideA, int a_kcontig,
+                            const double* B, int64_t ldb, int64_t strideB, int b_ncontig,
+                            double beta, double* C, int64_t ldc, int64_t strideC,
+                            int precision, tnx_stream_t stream) {
+  using namespace tnx;
+  TNX_CHECK_ARG(precision >= TNX_F64 && precision <= TNX_BF16X3, "tnx_gemm_f64: bad precision %d", precision);
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K; d.batch = batch;
+  d.A = A; d.ta = DT::F64; d.lda = lda; d.sA = strideA; d.a_kcontig = a_kcontig != 0;
+  d.B = B; d.tb = DT::F64; d.ldb = ldb; d.sB = strideB; d.b_ncontig = b_ncontig != 0;
+  d.C = C; d.tc = DT::F64; d.ldc = ldc; d.sC = strideC;
+  d.alpha = alpha; d.beta = beta; d.prec = (Prec)precision;
+  return gemm(d, (hipStream_t)stream);
+}

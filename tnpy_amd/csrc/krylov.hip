@@ -1,0 +1,789 @@
+// Lowest eigenpair of the one-site effective Hamiltonian on the device, and
+// the small dense symmetric eigensolver.
+//
+// Replaces primme.eigsh(which='SA', k=1) (tnpy/linalg.py:64-87, called at
+// tnpy/finite_dmrg.py:111) and np.linalg.eigh below exact_solver_dim
+// (tnpy/linalg.py:42-61, tnpy/finite_dmrg.py:108-110).
+//
+// Krylov method: Lanczos with full reorthogonalisation (classical
+// Gram-Schmidt, two passes) and thick restart keeping one Ritz vector. The
+// projected matrix stays tridiagonal; its lowest pair is found by one
+// wavefront (64-way multisection on Sturm counts, then inverse iteration).
+// All Krylov scalars live in device memory; the host reads one small status
+// block per iteration to decide convergence / restart.
+//
+// Mixed precision: the Krylov matvecs run in FP32 or BF16x3; every cycle
+// starts from an FP64 matvec on the current Ritz vector (Rayleigh quotient
+// and residual in FP64), and the next cycle's basis is [y, r/|r|]. The
+// returned eigenvalue, residual and H*y are FP64.
+//
+// Vector kernels are HBM bound: a CGS pass over J basis vectors of length N
+// reads (J+1)*N*8 bytes (+N*8 written when it updates w).
+#include "tnx_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tnx {
+
+int heff1_matvec(const double* L, const double* W, const double* R, const double* x,
+                 double* y, int chil, int Dl, int Dr, int d, int chir, int prec,
+                 void* ws, size_t ws_bytes, hipStream_t s);
+size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir);
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+constexpr int kMaxBasis = 64;        // Lanczos vectors kept before restart
+constexpr int kPartStride = kMaxBasis + 2;
+constexpr int kCgsBlocks = 512;      // blocks of a CGS pass (partials rows)
+
+// One classical Gram-Schmidt pass over basis V[0..J-1] (row stride ldv):
+//   if hsub:  w -= sum_j V[j] * hsub[j]               (in place)
+//   partial[blk][j] = sum_i V[j][i] w[i]   (j < J, if DOTS)
+//   partial[blk][JMAX... stored at index J] = sum_i w[i]^2
+template <int JMAX, bool DOTS>
+__global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict__ V,
+                                                       int64_t ldv, int J,
+                                                       double* __restrict__ w,
+                                                       const double* __restrict__ hsub,
+                                                       double* __restrict__ partials,
+                                                       int64_t N) {
+  __shared__ double hs[JMAX];
+  __shared__ double red[4][JMAX + 1];
+  if (threadIdx.x < JMAX) hs[threadIdx.x] = (hsub && (int)threadIdx.x < J) ? hsub[threadIdx.x] : 0.0;
+  __syncthreads();
+  double acc[JMAX];
+#pragma unroll
+  for (int j = 0; j < JMAX; ++j) acc[j] = 0.0;
+  double nrm = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+    double vv[JMAX];
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? V[(int64_t)j * ldv + i] : 0.0;
+    double wi = w[i];
+    if (hsub) {
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) wi -= vv[j] * hs[j];
+      w[i] = wi;
+    }
+    if (DOTS) {
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) acc[j] += vv[j] * wi;
+    }
+    nrm += wi * wi;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (DOTS) {
+#pragma unroll
+    for (int j = 0; j < JMAX; ++j) {
+      const double t = wave_sum(acc[j]);
+      if (lane == 0) red[wv][j] = t;
+    }
+  }
+  {
+    const double t = wave_sum(nrm);
+    if (lane == 0) red[wv][JMAX] = t;
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * kPartStride;
+  if ((int)threadIdx.x < J && DOTS)
+    out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+  if (threadIdx.x == 0) out[J] = red[0][JMAX] + red[1][JMAX] + red[2][JMAX] + red[3][JMAX];
+}
+
+// out[j] = sum over blocks of partials[blk][j], j <= J, in block order.
+__global__ void reduce_partials_kernel(const double* partials, int nblk, int J, double* out) {
+  const int j = threadIdx.x;
+  if (j > J) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partials[(int64_t)b * kPartStride + j];
+  out[j] = s;
+}
+
+static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double* hsub,
+                    bool dots, double* partials, double* out, int64_t N, hipStream_t s) {
+  const int nblk = (int)std::min<int64_t>(kCgsBlocks, (N + 255) / 256);
+  auto go = [&](auto jtag, auto dtag) {
+    constexpr int JM = decltype(jtag)::value;
+    constexpr bool DT_ = decltype(dtag)::value;
+    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_>), dim3(nblk), dim3(256), 0, s, V, ldv, J, w,
+                       hsub, partials, N);
+  };
+  using T8 = std::integral_constant<int, 8>;
+  using T16 = std::integral_constant<int, 16>;
+  using T32 = std::integral_constant<int, 32>;
+  using T64 = std::integral_constant<int, 64>;
+  using BT = std::integral_constant<bool, true>;
+  using BF = std::integral_constant<bool, false>;
+  TNX_CHECK_ARG(J >= 0 && J <= kMaxBasis, "cgs_pass: J=%d out of range", J);
+  if (J <= 8) { if (dots) go(T8{}, BT{}); else go(T8{}, BF{}); }
+  else if (J <= 16) { if (dots) go(T16{}, BT{}); else go(T16{}, BF{}); }
+  else if (J <= 32) { if (dots) go(T32{}, BT{}); else go(T32{}, BF{}); }
+  else { if (dots) go(T64{}, BT{}); else go(T64{}, BF{}); }
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(128), 0, s, partials, nblk, J, out);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
+// out[i] = sum_j c[j] V[j][i]   (Ritz vector from the basis)
+__global__ void vcomb_kernel(const double* __restrict__ V, int64_t ldv, int J,
+                             const double* __restrict__ c, double* __restrict__ out, int64_t N) {
+  __shared__ double cs[kMaxBasis + 1];
+  if ((int)threadIdx.x < J) cs[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < J; ++j) acc += cs[j] * V[(int64_t)j * ldv + i];
+    out[i] = acc;
+  }
+}
+
+// out = x / sqrt(*nrm2)  (unit vector; zero if the norm underflows)
+__global__ void normalize_kernel(const double* __restrict__ x, const double* __restrict__ nrm2,
+                                 double* __restrict__ out, int64_t N) {
+  const double n2 = *nrm2;
+  const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = x[i] * inv;
+}
+
+// out = (*a) * x + (*b) * z, scalars read from device memory
+__global__ void axpbz_dev_kernel(const double* __restrict__ a, const double* __restrict__ x,
+                                 const double* __restrict__ b, const double* __restrict__ z,
+                                 double* __restrict__ out, int64_t N) {
+  const double av = *a, bv = b ? *b : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = av * x[i] + (z ? bv * z[i] : 0.0);
+}
+
+__global__ void axpy_kernel(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+__global__ void scal_kernel(int64_t n, double a, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+static int vec_blocks(int64_t N) { return (int)std::min<int64_t>((N + 255) / 256, 4096); }
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal lowest eigenpair, one wavefront.
+// diag a[0..n-1], off-diagonal b[0..n-2]. Writes the unit eigenvector to
+// s[0..n-1] and status: st[0] = theta, st[1] = |bres * s[n-1]| (Lanczos
+// residual estimate, bres read from *bres if given), st[2] = max row sum
+// (estimate of ||T||, a lower bound of ||H||).
+// ---------------------------------------------------------------------------
+constexpr int kTriMax = 256;
+
+__device__ int sturm_count(const double* a, const double* b2, int n, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += (q < 0.0);
+  for (int i = 1; i < n; ++i) {
+    q = a[i] - x - b2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+  }
+  return cnt;
+}
+
+__global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ ad,
+                                                            const double* __restrict__ bd, int n,
+                                                            const double* __restrict__ bres,
+                                                            double* __restrict__ s,
+                                                            double* __restrict__ st) {
+  __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
+  __shared__ double u0[kTriMax], u1[kTriMax], u2[kTriMax], z[kTriMax];
+  __shared__ double shr[4];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n; i += 64) {
+    a[i] = ad[i];
+    const double bi = (i < n - 1) ? bd[i] : 0.0;
+    b[i] = bi;
+    b2[i] = bi * bi;
+  }
+  __syncthreads();
+  if (n == 1) {
+    if (lane == 0) {
+      s[0] = 1.0;
+      st[0] = a[0];
+      st[1] = bres ? fabs(*bres) : 0.0;
+      st[2] = fabs(a[0]);
+    }
+    return;
+  }
+  // Gershgorin bounds.
+  if (lane == 0) {
+    double lo = 1e308, amin = 1e308, tn = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0);
+      lo = fmin(lo, a[i] - r);
+      amin = fmin(amin, a[i]);
+      tn = fmax(tn, fabs(a[i]) + r);
+    }
+    shr[0] = lo; shr[1] = amin; shr[2] = tn;
+  }
+  __syncthreads();
+  double lo = shr[0], hi = shr[1];
+  const double tnorm = shr[2];
+  const double pivmin = fmax(tnorm, 1e-300) * 1e-300 + 1e-300;
+  const double eps = 2.220446049250313e-16;
+  // 64-way multisection: each lane evaluates the Sturm count at one point.
+  for (int it = 0; it < 12; ++it) {
+    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const int c = sturm_count(a, b2, n, x, pivmin);
+    const unsigned long long mask = __ballot(c >= 1);
+    if (mask == 0ull) {
+      lo = __shfl(x, 63, 64);
+    } else {
+      const int f = __ffsll((long long)mask) - 1;
+      const double xf = __shfl(x, f, 64);
+      const double xp = __shfl(x, f > 0 ? f - 1 : 0, 64);
+      if (f > 0) lo = xp;
+      hi = xf;
+    }
+  }
+  const double theta = 0.5 * (lo + hi);
+  // Inverse iteration on (T - theta I), Gaussian elimination with partial
+  // pivoting (lane 0), zero pivots replaced by eps*||T||.
+  if (lane == 0) {
+    const double tiny = eps * fmax(tnorm, 1e-300);
+    for (int i = 0; i < n; ++i) z[i] = 1.0 + 0.01 * (double)((i * 7919) % 13);
+    for (int rep = 0; rep < 3; ++rep) {
+      // current pivot row: (c0, c1, c2) at columns i, i+1, i+2
+      double c0 = a[0] - theta, c1 = b[0], c2 = 0.0, rc = z[0];
+      for (int i = 0; i < n - 1; ++i) {
+        const double d0 = b[i], d1 = a[i + 1] - theta, d2 = (i + 1 < n - 1) ? b[i + 1] : 0.0;
+        double rd = z[i + 1];
+        if (fabs(c0) >= fabs(d0)) {
+          if (c0 == 0.0) c0 = tiny;
+          const double f = d0 / c0;
+          u0[i] = c0; u1[i] = c1; u2[i] = c2; z[i] = rc;
+          c0 = d1 - f * c1; c1 = d2 - f * c2; c2 = 0.0; rc = rd - f * rc;
+        } else {
+          const double f = c0 / d0;
+          u0[i] = d0; u1[i] = d1; u2[i] = d2; z[i] = rd;
+          const double n0 = c1 - f * d1, n1 = c2 - f * d2;
+          c0 = n0; c1 = n1; c2 = 0.0; rc = rc - f * rd;
+        }
+      }
+      if (c0 == 0.0) c0 = tiny;
+      u0[n - 1] = c0; u1[n - 1] = 0.0; u2[n - 1] = 0.0; z[n - 1] = rc;
+      // back substitution
+      z[n - 1] = z[n - 1] / u0[n - 1];
+      if (n >= 2) z[n - 2] = (z[n - 2] - u1[n - 2] * z[n - 1]) / u0[n - 2];
+      for (int i = n - 3; i >= 0; --i) z[i] = (z[i] - u1[i] * z[i + 1] - u2[i] * z[i + 2]) / u0[i];
+      double nn = 0.0;
+      for (int i = 0; i < n; ++i) nn += z[i] * z[i];
+      const double inv = 1.0 / sqrt(nn);
+      for (int i = 0; i < n; ++i) z[i] *= inv;
+    }
+    // fix the sign: largest component positive (deterministic output)
+    int im = 0;
+    for (int i = 1; i < n; ++i) if (fabs(z[i]) > fabs(z[im])) im = i;
+    const double sg = z[im] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) s[i] = sg * z[i];
+    st[0] = theta;
+    st[1] = bres ? fabs(*bres * z[n - 1]) : 0.0;
+    st[2] = tnorm;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense symmetric tridiagonalisation (one workgroup) and back-transform.
+// A: n x n symmetric in global workspace (overwritten), n <= 256.
+// Householder reflectors H_k = I - tau_k v_k v_k^T act on indices k+1..n-1.
+// ---------------------------------------------------------------------------
+__device__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int i = 0; i < nw; ++i) t += red[i];
+  return t;
+}
+
+__global__ __launch_bounds__(256) void sytrd_kernel(double* __restrict__ A, int n,
+                                                   double* __restrict__ Vst,
+                                                   double* __restrict__ tau,
+                                                   double* __restrict__ dg,
+                                                   double* __restrict__ eg) {
+  __shared__ double v[kTriMax], p[kTriMax];
+  __shared__ double red[8];
+  const int t = threadIdx.x;
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    const double alpha = A[(int64_t)(k + 1) * n + k];
+    double part = 0.0;
+    for (int i = 1 + t; i < m; i += blockDim.x) {
+      const double x = A[(int64_t)(k + 1 + i) * n + k];
+      part += x * x;
+    }
+    const double sigma = block_sum(part, red);
+    double tk, beta;
+    if (sigma == 0.0) {
+      tk = 0.0; beta = alpha;
+      for (int i = t; i < m; i += blockDim.x) v[i] = (i == 0) ? 1.0 : 0.0;
+    } else {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tk = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int i = t; i < m; i += blockDim.x) v[i] = (i == 0) ? 1.0 : A[(int64_t)(k + 1 + i) * n + k] * sc;
+    }
+    __syncthreads();
+    for (int i = t; i < m; i += blockDim.x) Vst[(int64_t)k * n + i] = v[i];
+    if (t == 0) { tau[k] = tk; eg[k] = beta; }
+    if (tk != 0.0) {
+      // p = tau * A22 v
+      for (int i = t; i < m; i += blockDim.x) {
+        const double* row = A + (int64_t)(k + 1 + i) * n + (k + 1);
+        double acc = 0.0;
+        for (int j = 0; j < m; ++j) acc += row[j] * v[j];
+        p[i] = tk * acc;
+      }
+      __syncthreads();
+      double pv = 0.0;
+      for (int i = t; i < m; i += blockDim.x) pv += p[i] * v[i];
+      const double K = 0.5 * tk * block_sum(pv, red);
+      for (int i = t; i < m; i += blockDim.x) p[i] -= K * v[i];  // p <- w
+      __syncthreads();
+      for (int i = t; i < m; i += blockDim.x) {
+        double* row = A + (int64_t)(k + 1 + i) * n + (k + 1);
+        const double vi = v[i], wi = p[i];
+        for (int j = 0; j < m; ++j) row[j] -= vi * p[j] + wi * v[j];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < n; i += blockDim.x) dg[i] = A[(int64_t)i * n + i];
+  if (t == 0 && n >= 2) eg[n - 2] = A[(int64_t)(n - 1) * n + (n - 2)];
+}
+
+__global__ __launch_bounds__(256) void ormtr_kernel(const double* __restrict__ Vst,
+                                                   const double* __restrict__ tau, int n,
+                                                   const double* __restrict__ zin,
+                                                   double* __restrict__ x) {
+  __shared__ double xs[kTriMax];
+  __shared__ double red[8];
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += blockDim.x) xs[i] = zin[i];
+  __syncthreads();
+  for (int k = n - 3; k >= 0; --k) {
+    const double tk = tau[k];
+    if (tk == 0.0) continue;
+    const int m = n - k - 1;
+    double part = 0.0;
+    for (int i = t; i < m; i += blockDim.x) part += Vst[(int64_t)k * n + i] * xs[k + 1 + i];
+    const double dot = block_sum(part, red);
+    for (int i = t; i < m; i += blockDim.x) xs[k + 1 + i] -= tk * Vst[(int64_t)k * n + i] * dot;
+    __syncthreads();
+  }
+  for (int i = t; i < n; i += blockDim.x) x[i] = xs[i];
+}
+
+size_t syev_ws_bytes(int n) {
+  const size_t nn = (size_t)n * n;
+  return (2 * nn + 6 * (size_t)n + 16) * 8 + 256;
+}
+
+// Lowest eigenpair of a dense symmetric n x n matrix (device). eval written
+// to *eval_dev (device) or copied to host if eval_host is given.
+int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* ws,
+                size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(n >= 1 && n <= kTriMax, "syev_lowest: n=%d outside [1, %d]", n, kTriMax);
+  TNX_CHECK_ARG(M && evec, "syev_lowest: null pointer");
+  TNX_CHECK_ARG(ws && ws_bytes >= syev_ws_bytes(n), "syev_lowest: workspace too small");
+  double* A = (double*)ws;
+  double* Vst = A + (size_t)n * n;
+  double* tau = Vst + (size_t)n * n;
+  double* dg = tau + n;
+  double* eg = dg + n;
+  double* z = eg + n;
+  double* st = z + n + 8;
+  TNX_HIP(hipMemcpyAsync(A, M, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  if (n == 1) {
+    TNX_TRY(fill_f64(evec, 1, 1.0, s));
+    if (eval_host) {
+      TNX_HIP(hipMemcpyAsync(eval_host, M, sizeof(double), hipMemcpyDeviceToHost, s));
+      TNX_HIP(hipStreamSynchronize(s));
+    }
+    return TNX_OK;
+  }
+  hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, n,
+                     (const double*)nullptr, z, st);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec);
+  TNX_LAUNCH_CHECK();
+  if (eval_host) {
+    TNX_HIP(hipMemcpyAsync(eval_host, st, sizeof(double), hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipStreamSynchronize(s));
+  }
+  return TNX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos driver
+// ---------------------------------------------------------------------------
+struct KWork {
+  double* V;      // (m+1) x N basis
+  double* w;      // N
+  double* y;      // N
+  void* mv;       // matvec workspace
+  size_t mv_bytes;
+  double* partials;
+  double* h1; double* h2; double* h3;  // reductions (kPartStride each)
+  double* alpha; double* beta;         // tridiagonal, kMaxBasis + 2 each
+  double* s;                           // Ritz coefficients
+  double* st;                          // status (16)
+  double* scal;                        // scratch scalars (16)
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
+  const int m = std::max(3, std::min(max_basis, kMaxBasis));
+  const size_t N = (size_t)chil * d * chir;
+  size_t b = 0;
+  b += align256((size_t)(m + 1) * N * 8);
+  b += 2 * align256(N * 8);
+  b += align256(heff1_ws_bytes(chil, Dl, Dr, d, chir));
+  b += align256((size_t)kCgsBlocks * kPartStride * 8);
+  b += 3 * align256(kPartStride * 8);
+  b += 3 * align256((kMaxBasis + 2) * 8);
+  b += 2 * align256(16 * 8);
+  return b + 256;
+}
+
+static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  auto take = [&](size_t bytes) { char* r = p; p += align256(bytes); return r; };
+  k.V = (double*)take((size_t)(m + 1) * N * 8);
+  k.w = (double*)take(N * 8);
+  k.y = (double*)take(N * 8);
+  k.mv = take(mvb); k.mv_bytes = mvb;
+  k.partials = (double*)take((size_t)kCgsBlocks * kPartStride * 8);
+  k.h1 = (double*)take(kPartStride * 8);
+  k.h2 = (double*)take(kPartStride * 8);
+  k.h3 = (double*)take(kPartStride * 8);
+  k.alpha = (double*)take((kMaxBasis + 2) * 8);
+  k.beta = (double*)take((kMaxBasis + 2) * 8);
+  k.s = (double*)take((kMaxBasis + 2) * 8);
+  k.st = (double*)take(16 * 8);
+  k.scal = (double*)take(16 * 8);
+}
+
+// Finish a Lanczos column j: alpha[j] = h1[j] + h2[j], beta[j] = |w|,
+// V[j+1] = w / beta[j].
+__global__ void lanczos_finish_kernel(const double* h1, const double* h2, const double* h3,
+                                      int j, double* alpha, double* beta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    alpha[j] = h1[j] + h2[j];
+    beta[j] = sqrt(fmax(h3[j + 1], 0.0));
+  }
+}
+
+// Thick restart: alpha[0] = theta, beta[0] = beta[j] * s[j]
+__global__ void restart_scalars_kernel(const double* st, const double* s, int j, double* alpha,
+                                       double* beta, double* scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double bj = beta[j] * s[j];
+    alpha[0] = st[0];
+    beta[0] = bj;
+    scal[0] = st[0];   // theta
+    scal[1] = bj;      // coefficient of V[j+1] in H y
+  }
+}
+
+// Outer (FP64) step scalars: theta = y.(Hy) = h1[0] + h2[0] (both CGS
+// passes), rho^2 = h3[1] (norm of r), alpha[0] = theta, beta[0] = rho.
+__global__ void outer_scalars_kernel(const double* h1, const double* h2, const double* h3,
+                                     double* alpha, double* beta, double* scal, double* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double th = h1[0] + h2[0];
+    const double rho = sqrt(fmax(h3[1], 0.0));
+    alpha[0] = th; beta[0] = rho;
+    scal[0] = th; scal[2] = -th; scal[3] = 1.0;
+    st[4] = th; st[5] = rho;
+  }
+}
+
+struct EigCtx {
+  const double *L, *W, *R;
+  int chil, Dl, Dr, d, chir;
+  int64_t N;
+  hipStream_t s;
+  KWork k;
+  int matvecs = 0, matvecs64 = 0;
+};
+
+static int apply_h(EigCtx& c, const double* x, double* y, int prec) {
+  c.matvecs++;
+  if (prec == TNX_F64) c.matvecs64++;
+  return heff1_matvec(c.L, c.W, c.R, x, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, c.k.mv,
+                      c.k.mv_bytes, c.s);
+}
+
+static int read_status(EigCtx& c, double* host, int n) {
+  TNX_HIP(hipMemcpyAsync(host, c.k.st, sizeof(double) * n, hipMemcpyDeviceToHost, c.s));
+  TNX_HIP(hipStreamSynchronize(c.s));
+  return TNX_OK;
+}
+
+// FP64 outer step on the unit vector y (in k.y): w = H y, theta = y.w,
+// r = w - theta y (in k.w), rho = |r|. Afterwards V[0] = y, V[1] = r/rho,
+// alpha[0] = theta, beta[0] = rho. Returns theta, rho on the host.
+static int outer_step(EigCtx& c, double* theta, double* rho) {
+  KWork& k = c.k;
+  const int64_t N = c.N;
+  TNX_TRY(apply_h(c, k.y, k.w, TNX_F64));
+  // pass 1: h1[0] = y.w ; pass 2: w -= y*h1[0] -> r, h2[0] = y.r, |r|^2 ;
+  // pass 3: r -= y*h2[0], |r|^2 -> h3[1]
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, nullptr, true, k.partials, k.h1, N, c.s));
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h1, true, k.partials, k.h2, N, c.s));
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h2, false, k.partials, k.h3, N, c.s));
+  hipLaunchKernelGGL(outer_scalars_kernel, dim3(1), dim3(1), 0, c.s, k.h1, k.h2, k.h3,
+                     k.alpha, k.beta, k.scal, k.st);
+  TNX_LAUNCH_CHECK();
+  double h[8];
+  TNX_TRY(read_status(c, h, 8));
+  *theta = h[4];
+  *rho = h[5];
+  return TNX_OK;
+}
+
+int eigsh_heff1(const double* L, const double* W, const double* R, const double* v0,
+                double* vec, double* hvec, int chil, int Dl, int Dr, int d, int chir,
+                const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(opts && info, "eigsh: null opts/info");
+  TNX_CHECK_ARG(L && W && R && v0 && vec, "eigsh: null pointer");
+  TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dr > 0 && d > 0 && chir > 0, "eigsh: bad dims");
+  TNX_CHECK_ARG(opts->precision >= TNX_F64 && opts->precision <= TNX_BF16X3, "eigsh: bad precision");
+  TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh: negative tol");
+  const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
+  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_ws_bytes(chil, Dl, Dr, d, chir, m), "eigsh: workspace too small");
+  EigCtx c;
+  c.L = L; c.W = W; c.R = R;
+  c.chil = chil; c.Dl = Dl; c.Dr = Dr; c.d = d; c.chir = chir;
+  c.N = (int64_t)chil * d * chir;
+  c.s = s;
+  carve(c.k, ws, m, (size_t)c.N, heff1_ws_bytes(chil, Dl, Dr, d, chir));
+  KWork& k = c.k;
+  const int64_t N = c.N;
+  const int prec = opts->precision;
+  const int max_mv = opts->max_matvecs > 0 ? opts->max_matvecs : 10000;
+  const double tol = opts->tol;
+  const int nb = vec_blocks(N);
+  // Basis size cannot exceed the dimension (Krylov space exhausts at N).
+  const int mb = (int)std::min<int64_t>(m, std::max<int64_t>(N, 1));
+
+  // y = v0 / |v0|
+  TNX_TRY(cgs_pass(v0, N, 0, const_cast<double*>(v0), nullptr, false, k.partials, k.h3, N, s));
+  hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, v0, k.h3, k.y, N);
+  TNX_LAUNCH_CHECK();
+
+  double theta = 0.0, rho = 0.0, anorm = 0.0;
+  int restarts = 0;
+  bool converged = false;
+  bool have_hy = false;  // k.w holds r = H y - theta y for the final y
+  const bool mixed = prec != TNX_F64;
+
+  while (true) {
+    TNX_TRY(outer_step(c, &theta, &rho));
+    have_hy = true;
+    anorm = std::max(anorm, std::fabs(theta));
+    if (rho <= tol * anorm || rho == 0.0 || N == 1) {
+      converged = true;
+      break;
+    }
+    if (c.matvecs >= max_mv) break;
+    // V[0] = y, V[1] = r / rho
+    TNX_TRY(copy_f64(k.y, k.V, N, s));
+    hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, k.V + N, N);
+    TNX_LAUNCH_CHECK();
+    // Inner Lanczos cycle in `prec`, from column j = 1.
+    const double rho_outer = rho;
+    const double inner_tol = mixed ? std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer))
+                                   : tol * anorm;
+    int j = 1;
+    bool inner_done = false;
+    have_hy = false;  // the inner cycle reuses k.w
+    double est = rho, theta_in = theta;
+    while (!inner_done) {
+      double* vj = k.V + (int64_t)j * N;
+      TNX_TRY(apply_h(c, vj, k.w, prec));
+      const int J = j + 1;
+      TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
+      TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
+      TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
+      hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
+                         k.alpha, k.beta);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
+                         k.beta + j, k.s, k.st);
+      TNX_LAUNCH_CHECK();
+      // V[j+1] = w / beta[j]
+      hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
+                         k.V + (int64_t)(j + 1) * N, N);
+      TNX_LAUNCH_CHECK();
+      double h[4];
+      TNX_TRY(read_status(c, h, 3));
+      theta_in = h[0];
+      est = h[1];
+      anorm = std::max(anorm, h[2]);
+      const bool exhausted = (J >= N);
+      if (est <= inner_tol || exhausted || c.matvecs >= max_mv) {
+        // Ritz vector y = V[0..j] s
+        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N);
+        TNX_LAUNCH_CHECK();
+        inner_done = true;
+        if (!mixed && (est <= tol * anorm || exhausted)) {
+          // FP64 Lanczos: H y = theta y + beta_j s_j V[j+1]
+          hipLaunchKernelGGL(restart_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, j,
+                             k.alpha, k.beta, k.scal);
+          TNX_LAUNCH_CHECK();
+          if (hvec) {
+            hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal, k.y,
+                               exhausted ? nullptr : k.scal + 1, k.V + (int64_t)(j + 1) * N,
+                               hvec, N);
+            TNX_LAUNCH_CHECK();
+          }
+          theta = theta_in;
+          rho = est;
+          converged = true;
+        }
+      } else if (J >= mb) {
+        // thick restart: V[0] = y, V[1] = V[j+1]
+        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(restart_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, j,
+                           k.alpha, k.beta, k.scal);
+        TNX_LAUNCH_CHECK();
+        TNX_TRY(copy_f64(k.y, k.V, N, s));
+        TNX_TRY(copy_f64(k.V + (int64_t)(j + 1) * N, k.V + N, N, s));
+        restarts++;
+        j = 1;
+      } else {
+        j++;
+      }
+    }
+    if (converged) break;
+    if (c.matvecs >= max_mv) break;
+    // loop: next outer step re-evaluates y in FP64 (mixed precision, or
+    // FP64 when the matvec limit cut the cycle short)
+  }
+
+  // Outputs. After an outer step k.w holds r = H y - theta y.
+  if (have_hy) {
+    TNX_TRY(copy_f64(k.y, vec, N, s));
+    if (hvec) {
+      // H y = r + theta y  (scal[3] = 1, scal[0] = theta)
+      hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal + 3, k.w, k.scal,
+                         k.y, hvec, N);
+      TNX_LAUNCH_CHECK();
+    }
+  } else {
+    // FP64 Lanczos exit: hvec was written from the Lanczos relation.
+    TNX_TRY(copy_f64(k.y, vec, N, s));
+  }
+  TNX_HIP(hipStreamSynchronize(s));
+  info->eval = theta;
+  info->resnorm = rho;
+  info->anorm = anorm;
+  info->matvecs = c.matvecs;
+  info->matvecs_f64 = c.matvecs64;
+  info->restarts = restarts;
+  info->converged = converged ? 1 : 0;
+  return converged ? TNX_OK : TNX_ERR_NOCONV;
+}
+
+}  // namespace tnx
+
+// ---- C-ABI -----------------------------------------------------------------
+extern "C" {
+
+size_t tnx_syev_lowest_workspace_bytes(int n) { return tnx::syev_ws_bytes(n); }
+
+int tnx_syev_lowest(const double* M, int n, double* evec, double* eval_host, void* ws,
+                    size_t ws_bytes, tnx_stream_t stream) {
+  TNX_CHECK_ARG(eval_host, "tnx_syev_lowest: null eval_host");
+  return tnx::syev_lowest(M, n, evec, eval_host, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t tnx_eigsh_heff1_workspace_bytes(int chil, int Dl, int Dr, int d, int chir,
+                                       int max_basis) {
+  return tnx::eigsh_ws_bytes(chil, Dl, Dr, d, chir, max_basis);
+}
+
+int tnx_eigsh_heff1(const double* L, const double* W, const double* R, const double* v0,
+                    double* vec, double* hvec, int chil, int Dl, int Dr, int d, int chir,
+                    const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                    size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::eigsh_heff1(L, W, R, v0, vec, hvec, chil, Dl, Dr, d, chir, opts, info, ws,
+                          ws_bytes, (hipStream_t)stream);
+}
+
+int tnx_axpy(int64_t n, double alpha, const double* x, double* y, tnx_stream_t stream) {
+  using namespace tnx;
+  TNX_CHECK_ARG(n >= 0, "tnx_axpy: negative n");
+  if (n == 0) return TNX_OK;
+  TNX_CHECK_ARG(x && y, "tnx_axpy: null pointer");
+  hipLaunchKernelGGL(axpy_kernel, dim3(vec_blocks(n)), dim3(256), 0, (hipStream_t)stream, n,
+                     alpha, x, y);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
+int tnx_scal(int64_t n, double alpha, double* x, tnx_stream_t stream) {
+  using namespace tnx;
+  TNX_CHECK_ARG(n >= 0, "tnx_scal: negative n");
+  if (n == 0) return TNX_OK;
+  TNX_CHECK_ARG(x, "tnx_scal: null pointer");
+  hipLaunchKernelGGL(scal_kernel, dim3(vec_blocks(n)), dim3(256), 0, (hipStream_t)stream, n,
+                     alpha, x);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
+size_t tnx_dot_workspace_bytes(int64_t n) {
+  (void)n;
+  return (size_t)(tnx::kCgsBlocks + 2) * tnx::kPartStride * 8 + 256;
+}
+
+int tnx_dot(int64_t n, const double* x, const double* y, double* out_host, void* ws,
+            size_t ws_bytes, tnx_stream_t stream) {
+  using namespace tnx;
+  TNX_CHECK_ARG(n >= 1 && x && y && out_host, "tnx_dot: bad arguments");
+  TNX_CHECK_ARG(ws && ws_bytes >= tnx_dot_workspace_bytes(n), "tnx_dot: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  double* partials = (double*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* out = partials + (size_t)kCgsBlocks * kPartStride;
+  // one pass with V = x (J = 1), w = y: out[0] = x.y
+  TNX_TRY(cgs_pass(x, n, 1, const_cast<double*>(y), nullptr, true, partials, out, n, s));
+  TNX_HIP(hipMemcpyAsync(out_host, out, sizeof(double), hipMemcpyDeviceToHost, s));
+  TNX_HIP(hipStreamSynchronize(s));
+  return TNX_OK;
+}
+
+}  // extern "C"

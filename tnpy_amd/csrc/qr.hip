@@ -1,0 +1,425 @@
+// Blocked Householder QR and the untruncated gauge moves of one-site DMRG.
+//
+// The reference moves the orthogonality centre with an untruncated SVD
+// (tnpy/matrix_product_state.py:187-225 via tnpy/linalg.py:9-23): site <- U,
+// neighbour <- (s V^T) * neighbour. Any factorisation psi = Q X with Q an
+// isometry leaves the state unchanged and the site left-orthonormal; Q R is
+// the cheapest. Householder (not Cholesky) keeps Q orthonormal when psi is
+// rank deficient (product-like states, e.g. the ferromagnetic XXZ chain).
+//
+// Layout: A is row-major m x n (m >= n). Panels of NB columns are factored
+// by one workgroup that holds its rows in registers (two block reductions
+// per column: the column norm, then the reflector products with the other
+// panel columns fused with the V^T v Gram entries that build the compact-WY
+// factor T). Trailing updates and the explicit Q are GEMMs on MFMA:
+//   A_trail -= V (T^T (V^T A_trail)),   Q_sub -= V (T (V^T Q_sub)).
+#include "tnx_common.h"
+
+#include <algorithm>
+
+namespace tnx {
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum K per-thread values over the block; result in out[0..K-1] (LDS),
+// visible to all threads on return.
+template <int NT, int K>
+__device__ __forceinline__ void block_reduce(const double (&v)[K], double* red, double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t = wsum(v[k]);
+    if (lane == 0) red[wv * K + k] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w * K + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Factor the panel A[jb:m, jb:jb+b] (b <= NB). Writes R and the reflectors
+// back to A, the explicit V (unit diagonal, zeros above) to Vall at the same
+// positions, and the b x b upper-triangular T (row-major, ld NB) to Tp.
+template <int NT, int RPT, int NB>
+__global__ __launch_bounds__(NT) void qr_panel_kernel(double* __restrict__ A, int m, int n,
+                                                     int jb, int b, double* __restrict__ Vall,
+                                                     double* __restrict__ Tp) {
+  __shared__ double red[(NT / 64) * NB];
+  __shared__ double dots[NB];
+  __shared__ double sig[1];
+  __shared__ double alpha_s[1];
+  __shared__ double T[NB * NB];
+  const int tid = threadIdx.x;
+  const int mp = m - jb;
+  double P[RPT][NB];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int rho = tid + NT * r;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      P[r][c] = (rho < mp && c < b) ? A[(int64_t)(jb + rho) * n + jb + c] : 0.0;
+  }
+  for (int t = tid; t < NB * NB; t += NT) T[t] = 0.0;
+
+  // The column loop is a runtime loop; every access to P uses a compile-time
+  // column index (select on c == j) so the panel stays in registers.
+  for (int j = 0; j < b; ++j) {
+    // --- column norm below the diagonal, and alpha = P[j][j]
+    double part[1] = {0.0};
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int rho = tid + NT * r;
+      double pj = 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) pj = (c == j) ? P[r][c] : pj;
+      if (rho > j && rho < mp) part[0] += pj * pj;
+      if (rho == j) alpha_s[0] = pj;
+    }
+    block_reduce<NT, 1>(part, red, sig);
+    const double sigma = sig[0];
+    const double alpha = alpha_s[0];
+    double tau, beta, scale;
+    if (sigma == 0.0) {
+      tau = 0.0; beta = alpha; scale = 0.0;
+    } else {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    // v_j: 1 at rho == j, P*scale below (kept in column j); R diag = beta
+    double vj[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int rho = tid + NT * r;
+      const bool below = rho > j && rho < mp;
+      double pj = 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) pj = (c == j) ? P[r][c] : pj;
+      vj[r] = below ? pj * scale : (rho == j ? 1.0 : 0.0);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c == j) P[r][c] = below ? vj[r] : (rho == j ? beta : P[r][c]);
+    }
+    // --- one reduction: c > j: v_j . P[:, c] (reflector application);
+    //     c < j: v_j . v_c (Gram entries for T). Below the diagonal of a
+    //     column c < j, P[.][c] holds v_c, and rho >= j > c covers the
+    //     whole support of v_j.
+    {
+      // streamed: one wave reduction per column keeps a single partial live
+      const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        double pd = 0.0;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          const int rho = tid + NT * r;
+          if (c != j && c < b && rho >= j && rho < mp) pd += vj[r] * P[r][c];
+        }
+        pd = wsum(pd);
+        if (lane == 0) red[wv * NB + c] = pd;
+      }
+      __syncthreads();
+      if (tid < NB) {
+        double acc = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) acc += red[w * NB + tid];
+        dots[tid] = acc;
+      }
+      __syncthreads();
+    }
+    if (tau != 0.0) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c > j && c < b) {
+          const double f = tau * dots[c];
+#pragma unroll
+          for (int r = 0; r < RPT; ++r) {
+            const int rho = tid + NT * r;
+            if (rho >= j && rho < mp) P[r][c] -= f * vj[r];
+          }
+        }
+      }
+    }
+    // T[0:j, j] = -tau T[0:j, 0:j] y,  y_i = v_i . v_j ;  T[j][j] = tau
+    if (tid == 0) {
+      for (int i = 0; i < j; ++i) {
+        double acc = 0.0;
+        for (int l = i; l < j; ++l) acc += T[i * NB + l] * dots[l];
+        T[i * NB + j] = -tau * acc;
+      }
+      T[j * NB + j] = tau;
+    }
+    __syncthreads();
+  }
+  // write back R / reflectors, explicit V, and T
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int rho = tid + NT * r;
+    if (rho < mp) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < b) {
+          const int64_t off = (int64_t)(jb + rho) * n + jb + c;
+          A[off] = P[r][c];
+          Vall[off] = rho < c ? 0.0 : (rho == c ? 1.0 : P[r][c]);
+        }
+      }
+    }
+  }
+  for (int t = tid; t < NB * NB; t += NT) Tp[t] = T[t];
+}
+
+// R[i][c] = A[i][c] for c >= i, else 0   (n x n, row-major)
+__global__ void extract_r_kernel(const double* A, int n, double* R) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, c = e % n;
+    R[e] = c >= i ? A[i * n + c] : 0.0;
+  }
+}
+
+// A (m x n) <- [I_n ; 0]
+__global__ void eye_kernel(double* A, int m, int n) {
+  const int64_t mn = (int64_t)m * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, c = e % n;
+    A[e] = (i == c) ? 1.0 : 0.0;
+  }
+}
+
+struct QrCfg { int nt, rpt, nb; };
+
+static bool qr_config(int m, QrCfg* c) {
+  if (m <= 512) { *c = {512, 1, 16}; return true; }
+  if (m <= 1024) { *c = {512, 2, 16}; return true; }
+  if (m <= 2048) { *c = {512, 4, 16}; return true; }
+  if (m <= 4096) { *c = {1024, 4, 8}; return true; }
+  if (m <= 8192) { *c = {1024, 8, 4}; return true; }
+  return false;
+}
+
+static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t qr_ws_bytes(int m, int n) {
+  QrCfg c;
+  if (!qr_config(m, &c)) c = {1024, 8, 4};
+  const int np = (n + c.nb - 1) / c.nb;
+  size_t b = al((size_t)m * n * 8);                 // Vall
+  b += al((size_t)np * c.nb * c.nb * 8);            // T factors
+  b += 2 * al((size_t)c.nb * n * 8);                // W, W2
+  b += al((size_t)32 * c.nb * n * 8);               // split-K partials
+  return b + 256;
+}
+
+static int launch_panel(const QrCfg& c, double* A, int m, int n, int jb, int b, double* Vall,
+                        double* Tp, hipStream_t s) {
+#define TNX_PANEL(NT_, RPT_, NB_)                                                        \
+  if (c.nt == NT_ && c.rpt == RPT_ && c.nb == NB_) {                                     \
+    hipLaunchKernelGGL((qr_panel_kernel<NT_, RPT_, NB_>), dim3(1), dim3(NT_), 0, s, A, m, n, \
+                       jb, b, Vall, Tp);                                                 \
+    TNX_LAUNCH_CHECK();                                                                  \
+    return TNX_OK;                                                                       \
+  }
+  TNX_PANEL(512, 1, 16)
+  TNX_PANEL(512, 2, 16)
+  TNX_PANEL(512, 4, 16)
+  TNX_PANEL(1024, 4, 8)
+  TNX_PANEL(1024, 8, 4)
+#undef TNX_PANEL
+  set_error("qr: no panel configuration");
+  return TNX_ERR_UNSUPPORTED;
+}
+
+int qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(A && R, "qr: null pointer");
+  TNX_CHECK_ARG(m >= 1 && n >= 1 && m >= n, "qr: need m >= n >= 1 (m=%d n=%d)", m, n);
+  QrCfg c;
+  if (!qr_config(m, &c)) {
+    set_error("qr: m=%d rows exceeds the supported 8192", m);
+    return TNX_ERR_UNSUPPORTED;
+  }
+  TNX_CHECK_ARG(ws && ws_bytes >= qr_ws_bytes(m, n), "qr: workspace too small");
+  const int NB = c.nb;
+  const int np = (n + NB - 1) / NB;
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* Vall = (double*)p; p += al((size_t)m * n * 8);
+  double* Tall = (double*)p; p += al((size_t)np * NB * NB * 8);
+  double* Wb = (double*)p; p += al((size_t)NB * n * 8);
+  double* W2 = (double*)p; p += al((size_t)NB * n * 8);
+  double* part = (double*)p;
+  const size_t part_bytes = al((size_t)32 * NB * n * 8);
+
+  // ---- factorisation
+  for (int pi = 0; pi < np; ++pi) {
+    const int jb = pi * NB;
+    const int b = std::min(NB, n - jb);
+    const int mp = m - jb;
+    double* Tp = Tall + (size_t)pi * NB * NB;
+    TNX_TRY(launch_panel(c, A, m, n, jb, b, Vall, Tp, s));
+    const int nt = n - jb - b;
+    if (nt <= 0) continue;
+    const double* Vp = Vall + (size_t)jb * n + jb;
+    double* At = A + (size_t)jb * n + jb + b;
+    // W = V^T A_trail   (b x nt)
+    GemmDesc g{};
+    g.M = b; g.N = nt; g.K = mp; g.batch = 1;
+    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+    g.B = At; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+    g.C = Wb; g.tc = DT::F64; g.ldc = nt; g.sC = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+    TNX_TRY(gemm(g, s));
+    // W2 = T^T W
+    GemmDesc g2{};
+    g2.M = b; g2.N = nt; g2.K = b; g2.batch = 1;
+    g2.A = Tp; g2.ta = DT::F64; g2.lda = NB; g2.sA = 0; g2.a_kcontig = false;
+    g2.B = Wb; g2.tb = DT::F64; g2.ldb = nt; g2.sB = 0; g2.b_ncontig = true;
+    g2.C = W2; g2.tc = DT::F64; g2.ldc = nt; g2.sC = 0;
+    g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
+    TNX_TRY(gemm(g2, s));
+    // A_trail -= V W2
+    GemmDesc g3{};
+    g3.M = mp; g3.N = nt; g3.K = b; g3.batch = 1;
+    g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+    g3.B = W2; g3.tb = DT::F64; g3.ldb = nt; g3.sB = 0; g3.b_ncontig = true;
+    g3.C = At; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+    TNX_TRY(gemm(g3, s));
+  }
+  // ---- R, then Q = H_0 ... H_{np-1} [I; 0] accumulated backwards into A
+  {
+    const int64_t nn = (int64_t)n * n;
+    const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 4096);
+    hipLaunchKernelGGL(extract_r_kernel, dim3(blocks), dim3(256), 0, s, A, n, R);
+    TNX_LAUNCH_CHECK();
+    const int64_t mn = (int64_t)m * n;
+    const int b2 = (int)std::min<int64_t>((mn + 255) / 256, 4096);
+    hipLaunchKernelGGL(eye_kernel, dim3(b2), dim3(256), 0, s, A, m, n);
+    TNX_LAUNCH_CHECK();
+  }
+  for (int pi = np - 1; pi >= 0; --pi) {
+    const int jb = pi * NB;
+    const int b = std::min(NB, n - jb);
+    const int mp = m - jb;
+    const int nq = n - jb;
+    const double* Tp = Tall + (size_t)pi * NB * NB;
+    const double* Vp = Vall + (size_t)jb * n + jb;
+    double* Qs = A + (size_t)jb * n + jb;
+    GemmDesc g{};
+    g.M = b; g.N = nq; g.K = mp; g.batch = 1;
+    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+    g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+    g.C = Wb; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+    TNX_TRY(gemm(g, s));
+    GemmDesc g2{};
+    g2.M = b; g2.N = nq; g2.K = b; g2.batch = 1;
+    g2.A = Tp; g2.ta = DT::F64; g2.lda = NB; g2.sA = 0; g2.a_kcontig = true;
+    g2.B = Wb; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
+    g2.C = W2; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
+    g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
+    TNX_TRY(gemm(g2, s));
+    GemmDesc g3{};
+    g3.M = mp; g3.N = nq; g3.K = b; g3.batch = 1;
+    g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+    g3.B = W2; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+    g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+    TNX_TRY(gemm(g3, s));
+  }
+  return TNX_OK;
+}
+
+// ---- gauge moves -------------------------------------------------------------
+size_t gauge_ws_bytes(int chil, int d, int chir) {
+  const int64_t m1 = (int64_t)chil * d, n1 = chir;          // rightward
+  const int64_t m2 = (int64_t)d * chir, n2 = chil;          // leftward (transposed)
+  size_t a = 0, b = 0;
+  if (m1 >= n1) a = al((size_t)n1 * n1 * 8) + qr_ws_bytes((int)m1, (int)n1);
+  if (m2 >= n2) b = al((size_t)m2 * n2 * 8) + al((size_t)n2 * n2 * 8) + qr_ws_bytes((int)m2, (int)n2);
+  return std::max(a, b) + 256;
+}
+
+int gauge_right(double* A, const double* next, double* next_out, int chil, int d, int chir,
+                int next_cols, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(A && next && next_out && next != next_out, "gauge_right: bad pointers");
+  TNX_CHECK_ARG(chil > 0 && d > 0 && chir > 0 && next_cols > 0, "gauge_right: bad dims");
+  const int m = chil * d, n = chir;
+  TNX_CHECK_ARG(m >= n, "gauge_right: chil*d=%d < chir=%d (bond would shrink)", m, n);
+  TNX_CHECK_ARG(ws && ws_bytes >= gauge_ws_bytes(chil, d, chir), "gauge_right: workspace too small");
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* R = (double*)p; p += al((size_t)n * n * 8);
+  TNX_TRY(qr(A, m, n, R, p, qr_ws_bytes(m, n), s));
+  // next_out = R * next
+  GemmDesc g{};
+  g.M = n; g.N = next_cols; g.K = n; g.batch = 1;
+  g.A = R; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = true;
+  g.B = next; g.tb = DT::F64; g.ldb = next_cols; g.sB = 0; g.b_ncontig = true;
+  g.C = next_out; g.tc = DT::F64; g.ldc = next_cols; g.sC = 0;
+  g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+  return gemm(g, s);
+}
+
+int gauge_left(double* A, const double* prev, double* prev_out, int chil, int d, int chir,
+               int prev_rows, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(A && prev && prev_out && prev != prev_out, "gauge_left: bad pointers");
+  TNX_CHECK_ARG(chil > 0 && d > 0 && chir > 0 && prev_rows > 0, "gauge_left: bad dims");
+  const int m = d * chir, n = chil;
+  TNX_CHECK_ARG(m >= n, "gauge_left: d*chir=%d < chil=%d (bond would shrink)", m, n);
+  TNX_CHECK_ARG(ws && ws_bytes >= gauge_ws_bytes(chil, d, chir), "gauge_left: workspace too small");
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* At = (double*)p; p += al((size_t)m * n * 8);
+  double* R = (double*)p; p += al((size_t)n * n * 8);
+  // A (chil x d*chir) -> At (d*chir x chil); At = Q R; A = Q^T; prev <- prev R^T
+  TNX_TRY(transpose_f64(A, At, n, m, s));
+  TNX_TRY(qr(At, m, n, R, p, qr_ws_bytes(m, n), s));
+  TNX_TRY(transpose_f64(At, A, m, n, s));
+  GemmDesc g{};
+  g.M = prev_rows; g.N = n; g.K = n; g.batch = 1;
+  g.A = prev; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = true;
+  g.B = R; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = false;
+  g.C = prev_out; g.tc = DT::F64; g.ldc = n; g.sC = 0;
+  g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+  return gemm(g, s);
+}
+
+}  // namespace tnx
+
+extern "C" {
+
+size_t tnx_qr_workspace_bytes(int m, int n) { return tnx::qr_ws_bytes(m, n); }
+
+int tnx_qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::qr(A, m, n, R, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t tnx_gauge_workspace_bytes(int chil, int d, int chir) {
+  return tnx::gauge_ws_bytes(chil, d, chir);
+}
+
+int tnx_gauge_right(double* A, const double* next, double* next_out, int chil, int d, int chir,
+                    int next_cols, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::gauge_right(A, next, next_out, chil, d, chir, next_cols, ws, ws_bytes,
+                          (hipStream_t)stream);
+}
+
+int tnx_gauge_left(double* A, const double* prev, double* prev_out, int chil, int d, int chir,
+                   int prev_rows, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::gauge_left(A, prev, prev_out, chil, d, chir, prev_rows, ws, ws_bytes,
+                         (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Shared definitions of the tnx library: status codes, error reporting,
+// launch checks and small device helpers. Internal header (not part of the
+// C-ABI in include/tnx.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/tnx.h"
+
+namespace tnx {
+
+// Thread-local message of the last failing call on this host thread.
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define TNX_CHECK_ARG(cond, ...)                                   \
+  do {                                                             \
+    if (!(cond)) {                                                 \
+      ::tnx::set_error(__VA_ARGS__);                               \
+      return TNX_ERR_INVALID;                                      \
+    }                                                              \
+  } while (0)
+
+#define TNX_HIP(call)                                              \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) {                                        \
+      ::tnx::set_error("%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                        \
+      return TNX_ERR_HIP;                                          \
+    }                                                              \
+  } while (0)
+
+#define TNX_LAUNCH_CHECK()                                         \
+  do {                                                             \
+    hipError_t e_ = hipGetLastError();                             \
+    if (e_ != hipSuccess) {                                        \
+      ::tnx::set_error("kernel launch: %s (%s:%d)",                \
+                       hipGetErrorString(e_), __FILE__, __LINE__); \
+      return TNX_ERR_HIP;                                          \
+    }                                                              \
+  } while (0)
+
+#define TNX_TRY(expr)                                              \
+  do {                                                             \
+    int s_ = (expr);                                               \
+    if (s_ != TNX_OK) return s_;                                   \
+  } while (0)
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Precision of a contraction (matches tnx_precision_t in include/tnx.h).
+enum class Prec : int { F64 = TNX_F64, F32 = TNX_F32, BF16X3 = TNX_BF16X3 };
+
+// ---------------------------------------------------------------------------
+// Generic strided, batched GEMM on MFMA:
+//   C[b](m, n) = alpha * sum_k A[b](m, k) * B[b](k, n) + beta * C[b](m, n)
+// A(m, k) = a_kcontig ? A[m*lda + k] : A[k*lda + m]
+// B(k, n) = b_ncontig ? B[k*ldb + n] : B[n*ldb + k]
+// C(m, n) = C[m*ldc + n]                      (row-major output)
+// Element types: A/B are double or float, C is double or float.
+// ---------------------------------------------------------------------------
+enum class DT : int { F64 = 0, F32 = 1 };
+
+struct GemmDesc {
+  int M, N, K, batch;
+  const void* A; DT ta; int64_t lda; int64_t sA; bool a_kcontig;
+  const void* B; DT tb; int64_t ldb; int64_t sB; bool b_ncontig;
+  void* C;       DT tc; int64_t ldc; int64_t sC;
+  double alpha, beta;
+  Prec prec;
+  // Optional FP64 scratch for split-K partial sums (used when the output
+  // has few tiles and K is long); null disables splitting.
+  double* splitk_ws = nullptr;
+  size_t splitk_ws_bytes = 0;
+};
+
+int gemm(const GemmDesc& g, hipStream_t stream);
+
+// Elementwise kernels used by several modules.
+int transpose_f64(const double* in, double* out, int rows, int cols,
+                  hipStream_t stream);  // out[c][r] = in[r][c]
+int copy_f64(const double* in, double* out, int64_t n, hipStream_t stream);
+int fill_f64(double* out, int64_t n, double v, hipStream_t stream);
+
+}  // namespace tnx

@@ -1,0 +1,316 @@
+"""One-site finite DMRG on one MI355X (``tnpy/finite_dmrg.py:22-263``).
+
+Same public surface as the reference: ``FiniteDMRG(mpo, bond_dim)`` (or
+``chi=`` as in ``README.md:96-101``), ``run(tol, max_sweep, metric)`` with
+``update`` as its alias, ``sweep``, ``one_site_solver``,
+``perturb_wave_function``, ``mps``, ``variance``, ``measurements``.
+
+Per site the sweep runs, all on the GPU: the lowest eigenpair of H_eff
+(dense tridiagonalisation below ``exact_solver_dim``, thick-restart Lanczos
+otherwise), psi <- psi + 1e-5 H psi, the QR/LQ gauge move, and the
+environment update. One host synchronisation per site returns the energy.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+import warnings
+from datetime import timedelta
+from enum import Enum
+from functools import partial
+from itertools import cycle
+from typing import List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from tnpy_amd import _lib, logger
+from tnpy_amd.matrix_product_state import (
+    Direction,
+    Environment,
+    MatrixProductState,
+    MatrixProductStateMeasurements,
+)
+from tnpy_amd.operators import MatrixProductOperator, as_mpo
+
+
+class Metric(Enum):
+    """Which value the stopping criterion examines (``finite_dmrg.py:22-29``)."""
+
+    ENERGY = 1
+    VARIANCE = 2
+
+
+# primme keyword arguments the Lanczos solver can honour
+_KW_MAP = {"maxMatvecs": "max_matvecs", "maxBlockSize": None, "ncv": "max_basis", "maxBasisSize": "max_basis"}
+
+
+class FiniteDMRG:
+    """Finite-size one-site DMRG.
+
+    Args:
+        mpo: matrix product operator (``tnpy_amd.MatrixProductOperator``, a
+            list of W arrays, or a quimb MPO).
+        bond_dim: bond dimension of the random initial MPS (``chi`` is an
+            alias, as in the reference README).
+        block_size: accepted for signature compatibility; one-site only.
+        mps: optional initial state (``MatrixProductState`` or arrays).
+        exact_solver_dim: local dimension below which the dense eigensolver
+            is used (``finite_dmrg.py:108-110``).
+        precision: arithmetic of the Krylov matvecs, ``'fp64'``, ``'fp32'``
+            or ``'bf16x3'``; energies, residuals, environments and the gauge
+            moves stay FP64.
+        seed: seed of the random initial MPS.
+        max_basis: Lanczos basis size before a thick restart.
+        compute_variance: evaluate the variance after every sweep as the
+            reference's ``run`` does (``finite_dmrg.py:248``).
+    """
+
+    def __init__(
+        self,
+        mpo,
+        bond_dim: Optional[int] = None,
+        block_size: int = 1,
+        mps=None,
+        exact_solver_dim: int = 200,
+        *,
+        chi: Optional[int] = None,
+        precision: str = "fp64",
+        seed: Optional[int] = None,
+        max_basis: int = 32,
+        compute_variance: bool = True,
+        device: Optional[torch.device] = None,
+    ):
+        if chi is not None and bond_dim is not None and chi != bond_dim:
+            raise ValueError("chi and bond_dim disagree")
+        bond = chi if chi is not None else bond_dim
+        if bond is None:
+            raise TypeError("FiniteDMRG needs a bond dimension (bond_dim or chi)")
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+        self.device = device if device is not None else _lib.require_gpu()
+        _lib.load_library()
+        mpo = as_mpo(mpo)
+        self._n_sites = mpo.n_sites
+        self._bond_dim = int(bond)
+        self._phys_dim = mpo.phys_dim
+        self._block_size = block_size
+        self._exact_solver_dim = exact_solver_dim
+        self._precision = precision
+        self._max_basis = max_basis
+        self._compute_variance = compute_variance
+        if mps is None:
+            mps = MatrixProductState.random(
+                n=self.n_sites, bond_dim=self.bond_dim, phys_dim=self.phys_dim, seed=seed, device=self.device
+            )
+        elif not isinstance(mps, MatrixProductState):
+            mps = MatrixProductState(mps, device=self.device)
+        self._env = Environment(mpo=mpo, mps=mps)
+        self._energies: List[float] = [np.nan]
+        self._variances: List[float] = [np.nan]
+        self.site_energies: List[float] = []
+        self.stats = {"matvecs": 0, "matvecs_f64": 0, "restarts": 0, "dense_sites": 0, "krylov_sites": 0}
+        self._warned_kwargs = set()
+
+    # -- reference properties ----------------------------------------------------
+    @property
+    def bond_dim(self) -> int:
+        return self._bond_dim
+
+    @property
+    def mps(self) -> MatrixProductState:
+        return self._env.mps
+
+    @property
+    def n_sites(self) -> int:
+        return self._n_sites
+
+    @property
+    def phys_dim(self) -> int:
+        return self._phys_dim
+
+    @property
+    def precision(self) -> str:
+        return self._precision
+
+    @property
+    def environment(self) -> Environment:
+        return self._env
+
+    def variance(self) -> float:
+        return self._env.variance()
+
+    # -- local solver ------------------------------------------------------------
+    def _opts(self, tol: float, kwargs) -> _lib.EigshOpts:
+        o = _lib.EigshOpts()
+        o.tol = float(tol)
+        o.max_basis = int(self._max_basis)
+        o.max_matvecs = 5000
+        o.precision = _lib.PRECISIONS[self._precision]
+        for k, v in kwargs.items():
+            tgt = _KW_MAP.get(k, "?")
+            if tgt and tgt != "?":
+                setattr(o, tgt, int(v))
+            elif k not in self._warned_kwargs:
+                self._warned_kwargs.add(k)
+                warnings.warn(f"eigensolver keyword {k!r} has no effect on the GPU Lanczos solver")
+        return o
+
+    def _solve(self, site: int, tol: float, **kwargs) -> Tuple[float, torch.Tensor, torch.Tensor]:
+        env = self._env
+        A = env.mps.tensors[site]
+        N = A.numel()
+        lib = _lib.lib()
+        stream = _lib.stream_ptr(self.device)
+        if site > 0 and site not in env.left:
+            env.ensure_left(site)
+        l, Dl, Dr, d, r = env.site_dims(site)
+        vec = torch.empty(N, dtype=torch.float64, device=self.device)
+        hvec = torch.empty(N, dtype=torch.float64, device=self.device)
+        if N < self._exact_solver_dim:
+            # dense H_eff + symmetric eigensolver (finite_dmrg.py:108-110)
+            M = env.full_matrix_device(site)
+            wp, wb = env.workspace.get(lib.tnx_syev_lowest_workspace_bytes(N))
+            ev = C.c_double(0.0)
+            _lib.check(
+                lib.tnx_syev_lowest(_lib.ptr(M, "M"), N, _lib.ptr(vec, "evec"), C.byref(ev), wp, wb, stream),
+                "tnx_syev_lowest",
+            )
+            env.matvec_device(site, vec, hvec)
+            self.stats["dense_sites"] += 1
+            return float(ev.value), vec, hvec
+        opts = self._opts(tol, kwargs)
+        info = _lib.EigshInfo()
+        wp, wb = env.workspace.get(lib.tnx_eigsh_heff1_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis))
+        st = lib.tnx_eigsh_heff1(
+            _lib.ptr(env.left[site], "L"), _lib.ptr(env._W[site], "W"), _lib.ptr(env.right[site], "R"),
+            _lib.ptr(A.reshape(-1), "v0"), _lib.ptr(vec, "vec"), _lib.ptr(hvec, "hvec"),
+            l, Dl, Dr, d, r, C.byref(opts), C.byref(info), wp, wb, stream,
+        )
+        if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
+            _lib.check(st, "tnx_eigsh_heff1")
+        if st == _lib.TNX_ERR_NOCONV:
+            logger.warning(
+                f"Lanczos at site {site} stopped after {info.matvecs} matvecs with residual "
+                f"{info.resnorm:.3e} (tol {tol:.1e} x |H| {info.anorm:.3e})."
+            )
+        self.stats["matvecs"] += info.matvecs
+        self.stats["matvecs_f64"] += info.matvecs_f64
+        self.stats["restarts"] += info.restarts
+        self.stats["krylov_sites"] += 1
+        return float(info.eval), vec, hvec
+
+    def one_site_solver(self, site: int, tol: float = 1e-8, **kwargs) -> Tuple[float, np.ndarray]:
+        """Lowest eigenpair of the one-site H_eff (``finite_dmrg.py:97-111``);
+        returns the energy and the unit eigenvector as a host column."""
+        e, vec, _ = self._solve(site, tol, **kwargs)
+        return e, vec.cpu().numpy().reshape(-1, 1)
+
+    def two_site_solver(self, site: int, tol: float = 1e-8, **kwargs):
+        return NotImplemented
+
+    def perturb_wave_function(self, site: int, alpha: float = 1e-5) -> None:
+        """psi <- psi + alpha H_eff psi in place, not renormalised
+        (``finite_dmrg.py:116-141``)."""
+        psi = self.mps.tensors[site].reshape(-1).contiguous()
+        hpsi = torch.empty_like(psi)
+        self._env.matvec_device(site, psi, hpsi)
+        _lib.check(
+            _lib.lib().tnx_axpy(psi.numel(), alpha, _lib.ptr(hpsi, "hpsi"), _lib.ptr(psi, "psi"),
+                                _lib.stream_ptr(self.device)),
+            "tnx_axpy",
+        )
+        self.mps.set_site(site, psi)
+
+    def _site_step(self, site: int, direction: Direction, tol: float, **kwargs) -> float:
+        energy, vec, hvec = self._solve(site, tol, **kwargs)
+        logger.info(f"Sweeping to site [{site + 1}/{self.n_sites}], E0 = {energy}")
+        # update_mps + perturb_wave_function: psi = vec + 1e-5 H vec
+        _lib.check(
+            _lib.lib().tnx_axpy(vec.numel(), 1e-5, _lib.ptr(hvec, "hvec"), _lib.ptr(vec, "vec"),
+                                _lib.stream_ptr(self.device)),
+            "tnx_axpy",
+        )
+        self._env.update_mps(site, vec)
+        self._env.split_tensor(site, direction)
+        self._env.update(site, direction)
+        self.site_energies.append(energy)
+        return energy
+
+    def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8, **kwargs) -> Optional[float]:
+        """One directional sweep (``finite_dmrg.py:143-171``); returns the
+        energy at its last site."""
+        if direction == Direction.LEFTWARD:
+            self._env.ensure_left(self.n_sites - 1)
+        iter_sites = (
+            range(self.n_sites - 1) if direction == Direction.RIGHTWARD else range(self.n_sites - 1, 0, -1)
+        )
+        energy = None
+        for site in iter_sites:
+            energy = self._site_step(site, direction, tol, **kwargs)
+        return energy
+
+    def _converged(self, n_sweep: int, tol: float, max_sweep: int, metric: Metric) -> bool:
+        """Stopping rule and warnings of ``finite_dmrg.py:173-212``."""
+
+        def stopping_criterion(gradient: float) -> bool:
+            logger.info(f"Metric {metric.name} is lowered by {gradient:e} in this sweep.")
+            if abs(gradient) < tol:
+                logger.info(f"Reaching set tolerance {tol}, stop sweeping.")
+                return True
+            elif n_sweep == max_sweep:
+                logger.warning(
+                    f"Maximum number of sweeps {max_sweep} is reached, "
+                    f"yet {metric.name} gradient = {gradient:e} is still "
+                    f"greater than tol = {tol}."
+                )
+            elif abs(gradient) > tol and gradient < 0:
+                logger.warning(
+                    f"Might be trapped in local minimum in this sweep, "
+                    f"got {metric.name} gradient = {gradient:e}, skip and proceed."
+                )
+            return False
+
+        series = self._variances if metric == Metric.VARIANCE else self._energies
+        return stopping_criterion(np.diff(series[-2:])[0])
+
+    def run(self, tol: float = 1e-8, max_sweep: int = 100, metric: Metric = Metric.ENERGY, **kwargs) -> List[float]:
+        """Sweep until the metric changes by less than ``tol`` or
+        ``max_sweep`` is reached (``finite_dmrg.py:214-257``); returns the
+        energy of every sweep."""
+        if metric == Metric.VARIANCE and not self._compute_variance:
+            raise ValueError("Metric.VARIANCE needs compute_variance=True")
+        clock = [time.perf_counter()]
+        converged = partial(self._converged, tol=tol, max_sweep=max_sweep, metric=metric)
+        logger.info(f"Set tolerance = {tol} to metric {metric.name}, up to maximally {max_sweep} sweeps.")
+        n_sweep = 0
+        for n_sweep, direction in zip(range(1, max_sweep + 1), cycle([Direction.RIGHTWARD, Direction.LEFTWARD])):
+            logger.info(f"<==== In sweep epoch [{n_sweep}/{max_sweep}] ====>")
+            energy = self.sweep(direction, tol=tol, **kwargs)
+            torch.cuda.synchronize(self.device)
+            clock.append(time.perf_counter())
+            self._energies.append(energy)
+            self._variances.append(self._env.variance() if self._compute_variance else np.nan)
+            logger.info(f"Last sweep took {timedelta(seconds=np.diff(clock[-2:])[0])}.")
+            if converged(n_sweep):
+                break
+        self.sweep_times = list(np.diff(clock))
+        elapsed = np.mean(np.sort(np.diff(clock))[:3])
+        logger.info(f"Summary - {n_sweep} sweeps, best of {min(3, n_sweep)} - {timedelta(seconds=elapsed)} per sweep.")
+        return self._energies[1:]
+
+    update = run
+
+    @property
+    def energies(self) -> List[float]:
+        return self._energies[1:]
+
+    @property
+    def variances(self) -> List[float]:
+        return self._variances[1:]
+
+    @property
+    def measurements(self) -> MatrixProductStateMeasurements:
+        if len(self._energies) == 1:
+            raise RuntimeError("FiniteDMRG is probably not executed yet.")
+        return MatrixProductStateMeasurements(self.mps)
