@@ -15,8 +15,23 @@ from oracle import dmrg_ref as o
 pytestmark = pytest.mark.gpu
 
 
+_KEEP = []
+
+
 def dev(a):
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    """Device copy that stays alive until the test ends (a bare
+    ``dev(x).data_ptr()`` would let the caching allocator reuse the block
+    before the kernel reads it)."""
+    t = torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    _KEEP.append(t)
+    return t
+
+
+@pytest.fixture(autouse=True)
+def _release():
+    yield
+    torch.cuda.synchronize()
+    _KEEP.clear()
 
 
 def host(t):
@@ -32,6 +47,7 @@ def L():
 
 def ws(nbytes):
     buf = torch.empty(int(nbytes) + 256, dtype=torch.uint8, device="cuda")
+    _KEEP.append(buf)
     return buf, buf.data_ptr(), buf.numel()
 
 

@@ -216,7 +216,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __syncthreads();
   }
 
-  // Epilogue: D(i, j) of a 16x16 tile sits in lane j + 16*(i/4), item i%4.
+  // Epilogue. 16x16 accumulator tile, lane group lk = lane/16, item v:
+  //   f32 MFMAs: row = 4*lk + v ;  f64 MFMA: row = lk + 4*v ; column = lane%16
+  auto row_of = [&](int v) { return (PREC == 0) ? (lk + 4 * v) : (lk * 4 + v); };
   if (g.partial) {
     double* P = g.partial + ((size_t)bat * gridDim.z + split) * (size_t)g.M * g.N;
 #pragma unroll
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       for (int j = 0; j < TN; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int m = m0 + wm0 + i * 16 + lk * 4 + v;
+          const int m = m0 + wm0 + i * 16 + row_of(v);
           const int n = n0 + wn0 + j * 16 + lr;
           if (m < g.M && n < g.N) P[(size_t)m * g.N + n] = (double)acc[i][j][v];
         }
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int m = m0 + wm0 + i * 16 + lk * 4 + v;
+        const int m = m0 + wm0 + i * 16 + row_of(v);
         const int n = n0 + wn0 + j * 16 + lr;
         if (m < g.M && n < g.N) {
           const int64_t off = (int64_t)m * g.ldc + n;
