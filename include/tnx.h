@@ -101,7 +101,8 @@ int tnx_syev_lowest(const double* M, int n, double* evec, double* eval_host,
                     void* ws, size_t ws_bytes, tnx_stream_t stream);
 
 /* ---- Krylov lowest eigenpair of the one-site H_eff ---------------------
- * Thick-restart Lanczos with full CGS2 reorthogonalisation. Converged when
+ * Thick-restart Lanczos with full CGS2 reorthogonalisation, keeping the
+ * `keep` lowest Ritz vectors at a restart (clustered spectra). Converged when
  * the residual norm ||H y - theta y|| <= tol * ||H||_est (||H||_est = largest
  * |Ritz value| seen, PRIMME's criterion). With precision != TNX_F64 the
  * Krylov matvecs run in that precision and the Ritz pair is corrected by
@@ -115,7 +116,7 @@ typedef struct {
   int max_basis;      /* Lanczos basis size before a thick restart (>= 3) */
   int max_matvecs;    /* iteration limit (all precisions) */
   int precision;      /* TNX_F64 / TNX_F32 / TNX_BF16X3 for the Krylov matvecs */
-  int reserved;
+  int keep;           /* Ritz vectors kept at a thick restart (0 = default) */
 } tnx_eigsh_opts_t;
 
 typedef struct {

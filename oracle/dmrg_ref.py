@@ -290,6 +290,7 @@ def right_canonicalize(arrays: List[np.ndarray]) -> List[np.ndarray]:
         arrays[i] = q.T.reshape(l, p, r)
         pl, pp, pr = arrays[i - 1].shape
         arrays[i - 1] = (arrays[i - 1].reshape(pl * pp, pr) @ rr.T).reshape(pl, pp, pr)
+        arrays[i - 1] /= np.linalg.norm(arrays[i - 1])  # rescaling keeps long chains finite
     arrays[0] = arrays[0] / np.linalg.norm(arrays[0])
     return arrays
 

@@ -311,3 +311,49 @@ def test_blas1(gpu):
     out2 = C.c_double()
     check(L().tnx_dot(x.size, xd.data_ptr(), yd.data_ptr(), C.byref(out2), wp, wb, stream()))
     assert out.value == out2.value
+
+
+def clustered_env():
+    """Ferromagnetic XXZ(n=16) after two oracle sweeps and a partial third:
+    H_eff at site 8 has a cluster of low eigenvalues ~4e-4 apart on a
+    spread of ~10 (the local problem of config-3-like chains)."""
+    n, chi, site = 16, 16, 8
+    mpo = o.xxz_mpo(n, 1.0)
+    d = o.OracleDMRG(mpo, o.random_mps(n, chi, 2, seed=3))
+    d.sweep(o.RIGHTWARD, 1e-10)
+    d.sweep(o.LEFTWARD, 1e-10)
+    for s in range(site):
+        e, psi = d.one_site_solver(s, 1e-10)
+        d.mps[s] = psi.reshape(d.mps[s].shape)
+        d.perturb(s)
+        o.split_tensor_svd(d.mps, s, o.RIGHTWARD)
+        d.update_left(s + 1)
+    return d.left[site], mpo[site], d.right[site], d.mps[site]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("max_basis", [10, 32])
+def test_eigsh_clustered(gpu, prec, max_basis):
+    from tnpy_amd import _lib
+
+    print(f"eigsh clustered {prec} m={max_basis}", flush=True)
+    Lt, W, Rt, A = clustered_env()
+    M = o.heff_dense(Lt, W, Rt)
+    w = np.linalg.eigvalsh(M)
+    assert w[1] - w[0] < 1e-3  # the cluster is there
+    shape = (Lt.shape[0], W.shape[0], W.shape[1], W.shape[2], Rt.shape[0])
+    N = A.size
+    vec = torch.empty(N, dtype=torch.float64, device="cuda")
+    hvec = torch.empty(N, dtype=torch.float64, device="cuda")
+    opts = _lib.EigshOpts(tol=1e-10, max_basis=max_basis, max_matvecs=4000, precision=PREC[prec])
+    info = _lib.EigshInfo()
+    _, wp, wb = ws(L().tnx_eigsh_heff1_workspace_bytes(*shape, max_basis))
+    check(L().tnx_eigsh_heff1(dev(Lt).data_ptr(), dev(W).data_ptr(), dev(Rt).data_ptr(), dev(A).data_ptr(),
+                              vec.data_ptr(), hvec.data_ptr(), *shape, C.byref(opts), C.byref(info), wp, wb,
+                              stream()), "eigsh")
+    nrm = np.abs(w).max()
+    print(f"  matvecs={info.matvecs} f64={info.matvecs_f64} restarts={info.restarts}", flush=True)
+    assert info.converged == 1 and info.restarts >= 1
+    assert abs(info.eval - w[0]) <= 1e-12 * nrm
+    v = host(vec)
+    assert np.linalg.norm(M @ v - w[0] * v) <= 2e-10 * nrm

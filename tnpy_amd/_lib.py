@@ -36,7 +36,7 @@ class EigshOpts(C.Structure):
         ("max_basis", C.c_int),
         ("max_matvecs", C.c_int),
         ("precision", C.c_int),
-        ("reserved", C.c_int),
+        ("keep", C.c_int),
     ]
 
 

@@ -9,12 +9,22 @@
 //            a*b ~ ahi*bhi + ahi*blo + alo*bhi with a = ahi + alo (bf16),
 //            FP32 accumulate, LDS tiles [BM][BK] so each lane reads its
 //            8 consecutive k values with one 16-byte LDS read.
+// K-tile depth: 32 for FP64 (one 16x16x4 FP64 MFMA is 64 cycles, so a
+// deeper tile hides the HBM latency of the next tile's loads at one wave per
+// SIMD), 16 for FP32, 32 for BF16X3.
 // Operands may be FP64 or FP32 in HBM (runtime, wave-uniform branch); they
 // are converted while staging the tile in LDS. Block = 256 threads as a 2x2
 // grid of 64-wide wavefronts, each wave owning a (BM/2)x(BN/2) sub-tile of
 // 16x16 MFMA accumulators. Global loads of tile t+1 are issued before the
 // MFMAs of tile t; LDS is double-buffered (one barrier per k-tile).
 #include "tnx_common.h"
+
+#ifndef TNX_BK64
+#define TNX_BK64 32
+#endif
+#ifndef TNX_BK32
+#define TNX_BK32 16
+#endif
 
 namespace tnx {
 
@@ -39,8 +49,8 @@ __device__ __forceinline__ double ld_any(const void* p, int t, int64_t i) {
 }
 
 template <int PREC> struct PT;
-template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; using accs_t = double; static constexpr int BK = 16; };
-template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 16; };
+template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; using accs_t = double; static constexpr int BK = TNX_BK64; };
+template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; using accs_t = float;  static constexpr int BK = TNX_BK32; };
 template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 32; };
 
 template <int PREC, int BM, int BN, bool A_K, bool B_N>

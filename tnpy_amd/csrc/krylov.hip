@@ -97,13 +97,18 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict_
   if (threadIdx.x == 0) out[J] = red[0][JMAX] + red[1][JMAX] + red[2][JMAX] + red[3][JMAX];
 }
 
-// out[j] = sum over blocks of partials[blk][j], j <= J, in block order.
-__global__ void reduce_partials_kernel(const double* partials, int nblk, int J, double* out) {
-  const int j = threadIdx.x;
-  if (j > J) return;
+// out[j] = sum over blocks of partials[blk][j], j <= J. One workgroup per
+// j; a fixed strided split and a fixed tree, so the sum is deterministic.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* partials, int nblk,
+                                                              int J, double* out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partials[(int64_t)b * kPartStride + j];
-  out[j] = s;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[(int64_t)b * kPartStride + j];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double* hsub,
@@ -127,7 +132,7 @@ static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double
   else if (J <= 32) { if (dots) go(T32{}, BT{}); else go(T32{}, BF{}); }
   else { if (dots) go(T64{}, BT{}); else go(T64{}, BF{}); }
   TNX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(128), 0, s, partials, nblk, J, out);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(J + 1), dim3(256), 0, s, partials, nblk, J, out);
   TNX_LAUNCH_CHECK();
   return TNX_OK;
 }
@@ -443,34 +448,171 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
 }
 
 // ---------------------------------------------------------------------------
+// Dense symmetric eigensolver for the projected matrix (n <= 64), one
+// workgroup: cyclic Jacobi with round-robin (circle) pair ordering, so the
+// n/2 rotations of a round act on disjoint planes and are applied together
+// (all row rotations, then all column rotations). Matrix and eigenvectors
+// stay in LDS. Writes eigenvalues ascending to w, the matching eigenvectors
+// as columns of S (row stride ldS), the lowest eigenvector to s, and the
+// Lanczos status: st[0] = lowest eigenvalue, st[1] = |bres * S[n-1][0]|,
+// st[2] = max |eigenvalue|.
+// ---------------------------------------------------------------------------
+constexpr int kJac = 64;
+
+__global__ __launch_bounds__(256) void jacobi_eig_kernel(const double* __restrict__ Tin, int ldT,
+                                                         int n, double* __restrict__ w,
+                                                         double* __restrict__ S, int ldS,
+                                                         double* __restrict__ s,
+                                                         const double* __restrict__ bres,
+                                                         double* __restrict__ st) {
+  __shared__ double A[kJac][kJac + 1];
+  __shared__ double V[kJac][kJac + 1];
+  __shared__ double cs[kJac / 2], sn[kJac / 2];
+  __shared__ int P[kJac / 2], Q[kJac / 2];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int m = n + (n & 1);  // even size; an odd n gets one decoupled dummy index
+  for (int e = tid; e < m * m; e += nt) {
+    const int i = e / m, j = e % m;
+    A[i][j] = (i < n && j < n) ? Tin[i * ldT + j] : 0.0;
+    V[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int half = m / 2;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    // off-diagonal and total Frobenius norms
+    double off = 0.0, tot = 0.0;
+    for (int e = tid; e < m * m; e += nt) {
+      const int i = e / m, j = e % m;
+      const double a = A[i][j] * A[i][j];
+      tot += a;
+      if (i != j) off += a;
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    off = wave_sum(off);
+    tot = wave_sum(tot);
+    __syncthreads();
+    if (lane == 0) { red[wv] = off; }
+    __syncthreads();
+    double offs = 0.0;
+    for (int i = 0; i < (nt >> 6); ++i) offs += red[i];
+    __syncthreads();
+    if (lane == 0) { red[wv] = tot; }
+    __syncthreads();
+    double tots = 0.0;
+    for (int i = 0; i < (nt >> 6); ++i) tots += red[i];
+    if (offs <= 1e-32 * tots || m == 2 && sweep > 0) break;
+    for (int r = 0; r < m - 1; ++r) {
+      if (tid < half) {
+        auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + r) % (m - 1); };
+        int p = pos(tid), q = pos(m - 1 - tid);
+        if (p > q) { const int t = p; p = q; q = t; }
+        const double apq = A[p][q];
+        double c = 1.0, sgn = 0.0;
+        if (apq != 0.0) {
+          const double th = (A[q][q] - A[p][p]) / (2.0 * apq);
+          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          c = 1.0 / sqrt(t * t + 1.0);
+          sgn = t * c;
+        }
+        P[tid] = p; Q[tid] = q; cs[tid] = c; sn[tid] = sgn;
+      }
+      __syncthreads();
+      for (int e = tid; e < half * m; e += nt) {  // rows: A <- J^T A
+        const int pr = e / m, k = e % m;
+        const int p = P[pr], q = Q[pr];
+        const double c = cs[pr], sg = sn[pr];
+        const double a = A[p][k], b = A[q][k];
+        A[p][k] = c * a - sg * b;
+        A[q][k] = sg * a + c * b;
+      }
+      __syncthreads();
+      for (int e = tid; e < half * m; e += nt) {  // columns: A <- A J, V <- V J
+        const int pr = e / m, k = e % m;
+        const int p = P[pr], q = Q[pr];
+        const double c = cs[pr], sg = sn[pr];
+        const double a = A[k][p], b = A[k][q];
+        A[k][p] = c * a - sg * b;
+        A[k][q] = sg * a + c * b;
+        const double va = V[k][p], vb = V[k][q];
+        V[k][p] = c * va - sg * vb;
+        V[k][q] = sg * va + c * vb;
+      }
+      __syncthreads();
+    }
+  }
+  // sort the n genuine eigenpairs by value (the dummy index n stays decoupled)
+  for (int i = tid; i < n; i += nt) {
+    const double di = A[i][i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[j][j];
+      rank += (dj < di) || (dj == di && j < i);
+    }
+    w[rank] = di;
+    for (int k = 0; k < n; ++k) S[k * ldS + rank] = V[k][i];
+  }
+  __syncthreads();
+  __threadfence_block();
+  if (tid == 0) {
+    // status from the sorted output (global memory written by this block)
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i) amax = fmax(amax, fabs(A[i][i]));
+    int imin = 0;
+    for (int i = 1; i < n; ++i) if (A[i][i] < A[imin][imin]) imin = i;
+    // fix the sign: largest component positive
+    int im = 0;
+    for (int k = 1; k < n; ++k) if (fabs(V[k][imin]) > fabs(V[im][imin])) im = k;
+    const double sg = V[im][imin] < 0.0 ? -1.0 : 1.0;
+    for (int k = 0; k < n; ++k) s[k] = sg * V[k][imin];
+    st[0] = A[imin][imin];
+    st[1] = bres ? fabs(*bres * V[n - 1][imin]) : 0.0;
+    st[2] = amax;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Lanczos driver
 // ---------------------------------------------------------------------------
 struct KWork {
   double* V;      // (m+1) x N basis
+  double* Y;      // keep x N restart vectors
   double* w;      // N
   double* y;      // N
   void* mv;       // matvec workspace
   size_t mv_bytes;
   double* partials;
   double* h1; double* h2; double* h3;  // reductions (kPartStride each)
-  double* alpha; double* beta;         // tridiagonal, kMaxBasis + 2 each
-  double* s;                           // Ritz coefficients
+  double* alpha; double* beta;         // tridiagonal part, kMaxBasis + 2 each
+  double* T;                           // dense projected matrix, kMaxBasis^2
+  double* S;                           // its eigenvectors, kMaxBasis^2
+  double* Cm;                          // restart combination, kMaxBasis^2
+  double* theta;                       // its eigenvalues, kMaxBasis
+  double* s;                           // Ritz coefficients of the lowest pair
   double* st;                          // status (16)
   double* scal;                        // scratch scalars (16)
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+static int keep_of(int m, int keep) {
+  int k = keep > 0 ? keep : m / 4 + 2;
+  return std::max(1, std::min(k, m - 2));
+}
+
 size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
   const int m = std::max(3, std::min(max_basis, kMaxBasis));
   const size_t N = (size_t)chil * d * chir;
   size_t b = 0;
   b += align256((size_t)(m + 1) * N * 8);
+  b += align256((size_t)(m - 2) * N * 8);
   b += 2 * align256(N * 8);
   b += align256(heff1_ws_bytes(chil, Dl, Dr, d, chir));
   b += align256((size_t)kCgsBlocks * kPartStride * 8);
   b += 3 * align256(kPartStride * 8);
   b += 3 * align256((kMaxBasis + 2) * 8);
+  b += 3 * align256((size_t)kMaxBasis * kMaxBasis * 8);
+  b += align256((kMaxBasis + 2) * 8);
   b += 2 * align256(16 * 8);
   return b + 256;
 }
@@ -479,6 +621,7 @@ static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
   char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
   auto take = [&](size_t bytes) { char* r = p; p += align256(bytes); return r; };
   k.V = (double*)take((size_t)(m + 1) * N * 8);
+  k.Y = (double*)take((size_t)(m - 2) * N * 8);
   k.w = (double*)take(N * 8);
   k.y = (double*)take(N * 8);
   k.mv = take(mvb); k.mv_bytes = mvb;
@@ -489,39 +632,129 @@ static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
   k.alpha = (double*)take((kMaxBasis + 2) * 8);
   k.beta = (double*)take((kMaxBasis + 2) * 8);
   k.s = (double*)take((kMaxBasis + 2) * 8);
+  k.T = (double*)take((size_t)kMaxBasis * kMaxBasis * 8);
+  k.S = (double*)take((size_t)kMaxBasis * kMaxBasis * 8);
+  k.Cm = (double*)take((size_t)kMaxBasis * kMaxBasis * 8);
+  k.theta = (double*)take((kMaxBasis + 2) * 8);
   k.st = (double*)take(16 * 8);
   k.scal = (double*)take(16 * 8);
 }
 
-// Finish a Lanczos column j: alpha[j] = h1[j] + h2[j], beta[j] = |w|,
-// V[j+1] = w / beta[j].
+// Finish a Lanczos column j: alpha[j] = h1[j] + h2[j], beta[j] = |w|, and
+// the same entries in the dense projected matrix T.
 __global__ void lanczos_finish_kernel(const double* h1, const double* h2, const double* h3,
-                                      int j, double* alpha, double* beta) {
+                                      int j, double* alpha, double* beta, double* T) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    alpha[j] = h1[j] + h2[j];
-    beta[j] = sqrt(fmax(h3[j + 1], 0.0));
+    const double a = h1[j] + h2[j];
+    const double b = sqrt(fmax(h3[j + 1], 0.0));
+    alpha[j] = a;
+    beta[j] = b;
+    T[j * kMaxBasis + j] = a;
+    if (j + 1 < kMaxBasis) {
+      T[j * kMaxBasis + j + 1] = b;
+      T[(j + 1) * kMaxBasis + j] = b;
+    }
   }
 }
 
-// Thick restart: alpha[0] = theta, beta[0] = beta[j] * s[j]
-__global__ void restart_scalars_kernel(const double* st, const double* s, int j, double* alpha,
-                                       double* beta, double* scal) {
+// H y = theta y + (beta_j s_j) V[j+1] for the lowest Ritz pair:
+// scal[0] = theta, scal[1] = beta_j * s_j
+__global__ void ritz_scalars_kernel(const double* st, const double* s, const double* beta, int j,
+                                    double* scal) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double bj = beta[j] * s[j];
-    alpha[0] = st[0];
-    beta[0] = bj;
-    scal[0] = st[0];   // theta
-    scal[1] = bj;      // coefficient of V[j+1] in H y
+    scal[0] = st[0];
+    scal[1] = beta[j] * s[j];
+  }
+}
+
+// Thick restart that keeps the projected matrix tridiagonal. The kept Ritz
+// pairs (theta_i, Y_i = V S[:, i]), i < k, satisfy H Y_i = theta_i Y_i +
+// c_i v_next with c_i = beta_j S[j][i]. A Lanczos run on diag(theta) started
+// from c / |c| gives an orthonormal Q (k x k') in which diag(theta) is
+// tridiagonal and only its first vector couples to v_next (with |c|). The new
+// basis is the reversed Q columns, so the coupling sits on the last kept
+// vector and the Lanczos recurrence continues from V[k'] = v_next.
+// Writes C (J x k', ld kMaxBasis) with V_new = V_old C, the new alpha/beta
+// (beta[k'-1] = |c|), the dense T, and k' to *kp_out. One wavefront.
+__global__ __launch_bounds__(64) void tr_restart_kernel(const double* __restrict__ theta,
+                                                        const double* __restrict__ S,
+                                                        const double* __restrict__ beta_in, int j,
+                                                        int k, int J, double* __restrict__ Cm,
+                                                        double* __restrict__ alpha,
+                                                        double* __restrict__ beta,
+                                                        double* __restrict__ T, int* kp_out) {
+  __shared__ double q[kMaxBasis][kMaxBasis];
+  __shared__ double a[kMaxBasis], b[kMaxBasis];
+  const int l = threadIdx.x;  // component index of the k-vectors
+  const double bj = beta_in[j];
+  const double cl = (l < k) ? bj * S[j * kMaxBasis + l] : 0.0;
+  const double th = (l < k) ? theta[l] : 0.0;
+  double tmax = fabs(th);
+  for (int o = 32; o > 0; o >>= 1) tmax = fmax(tmax, __shfl_xor(tmax, o, 64));
+  const double cn = sqrt(wave_sum(cl * cl));
+  double qi = (cn > 0.0) ? cl / cn : (l == 0 ? 1.0 : 0.0);
+  if (l < kMaxBasis) q[0][l] = (l < k) ? qi : 0.0;
+  int kp = 1;
+  double qprev = 0.0, bprev = 0.0;
+  for (int i = 0; i < k; ++i) {
+    double w = th * qi;
+    const double ai = wave_sum(qi * w);
+    w -= ai * qi + bprev * qprev;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int t = 0; t <= i; ++t) {
+        const double h = wave_sum(q[t][l] * w);
+        w -= h * q[t][l];
+      }
+    if (l < k) {} else w = 0.0;
+    if (l == 0) a[i] = ai;
+    const double bn = sqrt(wave_sum(w * w));
+    kp = i + 1;
+    if (i == k - 1 || !(bn > 1e-13 * fmax(tmax, 1e-300))) break;
+    if (l == 0) b[i] = bn;
+    qprev = qi;
+    bprev = bn;
+    qi = w / bn;
+    q[i + 1][l] = qi;
+  }
+  __syncthreads();
+  // reversed order: new index r holds q[kp-1-r]
+  for (int r = 0; r < kp; ++r) {
+    const int i = kp - 1 - r;
+    for (int row = l; row < J; row += 64) {
+      double acc = 0.0;
+      for (int t = 0; t < k; ++t) acc += S[row * kMaxBasis + t] * q[i][t];
+      Cm[row * kMaxBasis + r] = acc;
+    }
+  }
+  for (int e = l; e < kMaxBasis * kMaxBasis; e += 64) T[e] = 0.0;
+  __syncthreads();
+  if (l == 0) {
+    for (int r = 0; r < kp; ++r) {
+      const int i = kp - 1 - r;
+      alpha[r] = a[i];
+      T[r * kMaxBasis + r] = a[i];
+      const double br = (r < kp - 1) ? b[i - 1] : cn;
+      beta[r] = br;
+      T[r * kMaxBasis + r + 1] = br;
+      T[(r + 1) * kMaxBasis + r] = br;
+    }
+    *kp_out = kp;
   }
 }
 
 // Outer (FP64) step scalars: theta = y.(Hy) = h1[0] + h2[0] (both CGS
-// passes), rho^2 = h3[1] (norm of r), alpha[0] = theta, beta[0] = rho.
+// passes), rho^2 = h3[1] (norm of r). Resets the projected matrix to the
+// two-vector basis [y, r/rho].
 __global__ void outer_scalars_kernel(const double* h1, const double* h2, const double* h3,
-                                     double* alpha, double* beta, double* scal, double* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double th = h1[0] + h2[0];
-    const double rho = sqrt(fmax(h3[1], 0.0));
+                                     double* alpha, double* beta, double* scal, double* st,
+                                     double* T) {
+  const double th = h1[0] + h2[0];
+  const double rho = sqrt(fmax(h3[1], 0.0));
+  for (int e = threadIdx.x; e < kMaxBasis * kMaxBasis; e += blockDim.x) {
+    const int i = e / kMaxBasis, c = e % kMaxBasis;
+    T[e] = (i == 0 && c == 0) ? th : ((i + c == 1) ? rho : 0.0);
+  }
+  if (threadIdx.x == 0) {
     alpha[0] = th; beta[0] = rho;
     scal[0] = th; scal[2] = -th; scal[3] = 1.0;
     st[4] = th; st[5] = rho;
@@ -551,19 +784,17 @@ static int read_status(EigCtx& c, double* host, int n) {
 }
 
 // FP64 outer step on the unit vector y (in k.y): w = H y, theta = y.w,
-// r = w - theta y (in k.w), rho = |r|. Afterwards V[0] = y, V[1] = r/rho,
-// alpha[0] = theta, beta[0] = rho. Returns theta, rho on the host.
+// r = w - theta y (in k.w), rho = |r|. Sets alpha[0] = theta, beta[0] = rho
+// and T = [[theta, rho], [rho, .]]. Returns theta, rho on the host.
 static int outer_step(EigCtx& c, double* theta, double* rho) {
   KWork& k = c.k;
   const int64_t N = c.N;
   TNX_TRY(apply_h(c, k.y, k.w, TNX_F64));
-  // pass 1: h1[0] = y.w ; pass 2: w -= y*h1[0] -> r, h2[0] = y.r, |r|^2 ;
-  // pass 3: r -= y*h2[0], |r|^2 -> h3[1]
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, nullptr, true, k.partials, k.h1, N, c.s));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h1, true, k.partials, k.h2, N, c.s));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h2, false, k.partials, k.h3, N, c.s));
-  hipLaunchKernelGGL(outer_scalars_kernel, dim3(1), dim3(1), 0, c.s, k.h1, k.h2, k.h3,
-                     k.alpha, k.beta, k.scal, k.st);
+  hipLaunchKernelGGL(outer_scalars_kernel, dim3(1), dim3(256), 0, c.s, k.h1, k.h2, k.h3,
+                     k.alpha, k.beta, k.scal, k.st, k.T);
   TNX_LAUNCH_CHECK();
   double h[8];
   TNX_TRY(read_status(c, h, 8));
@@ -597,6 +828,7 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
   const int nb = vec_blocks(N);
   // Basis size cannot exceed the dimension (Krylov space exhausts at N).
   const int mb = (int)std::min<int64_t>(m, std::max<int64_t>(N, 1));
+  const int keep = keep_of(mb, opts->keep);
 
   // y = v0 / |v0|
   TNX_TRY(cgs_pass(v0, N, 0, const_cast<double*>(v0), nullptr, false, k.partials, k.h3, N, s));
@@ -607,12 +839,20 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
   int restarts = 0;
   bool converged = false;
   bool have_hy = false;  // k.w holds r = H y - theta y for the final y
-  const bool mixed = prec != TNX_F64;
+  // Krylov matvec precision of the current cycle: starts at `prec`, and
+  // escalates to FP64 when an FP64-corrected cycle fails to cut the true
+  // residual by 4x (the low-precision operator cannot resolve the local
+  // spectrum, e.g. clusters narrower than its rounding).
+  int cprec = prec;
+  double rho_prev = 0.0;
 
   while (true) {
     TNX_TRY(outer_step(c, &theta, &rho));
     have_hy = true;
     anorm = std::max(anorm, std::fabs(theta));
+    if (cprec != TNX_F64 && rho_prev > 0.0 && rho > 0.25 * rho_prev) cprec = TNX_F64;
+    rho_prev = rho;
+    const bool mixed = cprec != TNX_F64;
     if (rho <= tol * anorm || rho == 0.0 || N == 1) {
       converged = true;
       break;
@@ -629,16 +869,15 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
     int j = 1;
     bool inner_done = false;
     have_hy = false;  // the inner cycle reuses k.w
-    double est = rho, theta_in = theta;
     while (!inner_done) {
       double* vj = k.V + (int64_t)j * N;
-      TNX_TRY(apply_h(c, vj, k.w, prec));
+      TNX_TRY(apply_h(c, vj, k.w, cprec));
       const int J = j + 1;
       TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
       TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
       TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
       hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
-                         k.alpha, k.beta);
+                         k.alpha, k.beta, k.T);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
                          k.beta + j, k.s, k.st);
@@ -649,8 +888,8 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
       TNX_LAUNCH_CHECK();
       double h[4];
       TNX_TRY(read_status(c, h, 3));
-      theta_in = h[0];
-      est = h[1];
+      const double theta_in = h[0];
+      const double est = h[1];
       anorm = std::max(anorm, h[2]);
       const bool exhausted = (J >= N);
       if (est <= inner_tol || exhausted || c.matvecs >= max_mv) {
@@ -660,8 +899,8 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
         inner_done = true;
         if (!mixed && (est <= tol * anorm || exhausted)) {
           // FP64 Lanczos: H y = theta y + beta_j s_j V[j+1]
-          hipLaunchKernelGGL(restart_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, j,
-                             k.alpha, k.beta, k.scal);
+          hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, k.beta, j,
+                             k.scal);
           TNX_LAUNCH_CHECK();
           if (hvec) {
             hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal, k.y,
@@ -674,16 +913,30 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
           converged = true;
         }
       } else if (J >= mb) {
-        // thick restart: V[0] = y, V[1] = V[j+1]
-        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N);
+        // Thick restart keeping the `keep` lowest Ritz pairs in a basis
+        // where the projected matrix stays tridiagonal.
+        hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(256), 0, s, k.T, kMaxBasis, J,
+                           k.theta, k.S, kMaxBasis, k.s, k.beta + j, k.st);
         TNX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(restart_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, j,
-                           k.alpha, k.beta, k.scal);
+        int* kp_dev = reinterpret_cast<int*>(k.st + 8);
+        hipLaunchKernelGGL(tr_restart_kernel, dim3(1), dim3(64), 0, s, k.theta, k.S, k.beta, j,
+                           keep, J, k.Cm, k.alpha, k.beta, k.T, kp_dev);
         TNX_LAUNCH_CHECK();
-        TNX_TRY(copy_f64(k.y, k.V, N, s));
-        TNX_TRY(copy_f64(k.V + (int64_t)(j + 1) * N, k.V + N, N, s));
+        int kp = 0;
+        TNX_HIP(hipMemcpyAsync(&kp, kp_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+        TNX_HIP(hipStreamSynchronize(s));
+        TNX_CHECK_ARG(kp >= 1 && kp <= keep, "eigsh: restart kept %d vectors", kp);
+        GemmDesc g{};
+        g.M = kp; g.N = (int)N; g.K = J; g.batch = 1;
+        g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = 0; g.a_kcontig = false;
+        g.B = k.V; g.tb = DT::F64; g.ldb = N; g.sB = 0; g.b_ncontig = true;
+        g.C = k.Y; g.tc = DT::F64; g.ldc = N; g.sC = 0;
+        g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+        TNX_TRY(gemm(g, s));
+        TNX_TRY(copy_f64(k.Y, k.V, (int64_t)kp * N, s));
+        TNX_TRY(copy_f64(k.V + (int64_t)(j + 1) * N, k.V + (int64_t)kp * N, N, s));
         restarts++;
-        j = 1;
+        j = kp;
       } else {
         j++;
       }

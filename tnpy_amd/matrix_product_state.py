@@ -207,6 +207,11 @@ class MatrixProductState:
         stream = _lib.stream_ptr(self.device)
         for i in range(self.n_sites - 1, 0, -1):
             self._gauge_left(i, ws, stream)
+            if normalize:
+                # keep the absorbed factor O(1): a long chain of random
+                # tensors otherwise overflows before the final normalisation
+                t = self._tensors[i - 1]
+                self._tensors[i - 1] = (t / torch.linalg.vector_norm(t)).contiguous()
         if normalize:
             nrm = torch.linalg.vector_norm(self._tensors[0])
             self._tensors[0] = (self._tensors[0] / nrm).contiguous()
