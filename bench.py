@@ -9,13 +9,13 @@ update), as ``tnpy/finite_dmrg.py:143-171``.
 
 Run: W untimed warm-up sweeps in FP64 from the seeded state, then snapshot
 the state and time exactly K sweeps in FP64 and K sweeps in the mixed
-precision (BF16x3 Krylov matvecs with FP64 correction), each from the same
-snapshot, so |dE| between them is meaningful. ``value`` is the mixed path's
+precision, each from the same snapshot, so |dE| between them is meaningful. ``value`` is the mixed path's
 sweeps/s (FP32 Krylov matvecs with FP64 residual correction by default;
 BF16x3's ~1e-5 operator error cannot resolve this chain's clustered local
 spectra, so it escalates to FP64 and is slower here); the FP64 figure and
-matvec TFLOP/s of every precision are reported beside it. The dominant kernel's roofline comes from HIP-event
-timing of the matvec GEMM at its real shape on the library's stream.
+matvec TFLOP/s of every precision are reported beside it. The dominant
+kernel's roofline comes from HIP-event timing of the matvec GEMM at its real
+shape on the library's stream.
 
 The CPU baseline times the numpy FP64 restatement of the same site update
 (test oracle, scipy/OpenBLAS on the host cores) on a bounded sample and
@@ -180,8 +180,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(
-            f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU path; "
-            "see DESIGN.md for how the sweep would shard)",
+            f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the DMRG sweep here is "
+            "single-GPU; there is no multi-GPU path)",
             file=sys.stderr,
         )
         sys.exit(2)
