@@ -137,6 +137,26 @@ int tnx_eigsh_heff1(const double* L, const double* W, const double* R,
                     const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
                     void* ws, size_t ws_bytes, tnx_stream_t stream);
 
+/* ---- two-site effective Hamiltonian ------------------------------------
+ * y[l'][c1][c2][r'] = sum L[l][w][l'] W1[w][u][k1][c1] W2[u][v][k2][c2]
+ *                         R[r][v][r'] x[l][k1][k2][r]
+ * No reference function: FiniteDMRG.two_site_solver is a stub returning
+ * NotImplemented (tnpy/finite_dmrg.py:113-114); these entry points fill it. */
+size_t tnx_heff2_workspace_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir);
+int tnx_heff2_matvec(const double* L, const double* W1, const double* W2,
+                     const double* R, const double* x, double* y, int chil,
+                     int Dl, int Dm, int Dr, int d, int chir, int precision,
+                     void* ws, size_t ws_bytes, tnx_stream_t stream);
+/* Lowest eigenpair of the two-site H_eff; same solver and options as
+ * tnx_eigsh_heff1 (the two-site counterpart of tnpy/linalg.py:64-87). */
+size_t tnx_eigsh_heff2_workspace_bytes(int chil, int Dl, int Dm, int Dr, int d,
+                                       int chir, int max_basis);
+int tnx_eigsh_heff2(const double* L, const double* W1, const double* W2,
+                    const double* R, const double* v0, double* vec, double* hvec,
+                    int chil, int Dl, int Dm, int Dr, int d, int chir,
+                    const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
+                    void* ws, size_t ws_bytes, tnx_stream_t stream);
+
 /* ---- gauge moves (untruncated) ----------------------------------------
  * Rightward: A (chil*d x chir) = Q R; A <- Q (left-orthonormal);
  *   next_out (chir x next_cols) = R * next. Replaces

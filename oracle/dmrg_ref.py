@@ -530,6 +530,65 @@ class OracleDMRG:
         return self.energies[1:]
 
 
+def split_two_site_svd(theta: np.ndarray, chi: int, direction: int):
+    """Truncated SVD of the two-site tensor theta[l, k1, k2, r] (shape of
+    (l*d) x (d*r)) keeping min(chi, l*d, d*r) singular values. Rightward:
+    (U, diag(s) V^T); leftward: (U diag(s), V^T). Returns the two site
+    tensors and the singular values."""
+    l, d1, d2, r = theta.shape
+    u, s, vt = np.linalg.svd(theta.reshape(l * d1, d2 * r), full_matrices=False)
+    k = min(chi, l * d1, d2 * r)
+    u, s, vt = u[:, :k], s[:k], vt[:k, :]
+    if direction == RIGHTWARD:
+        return u.reshape(l, d1, k), (s[:, None] * vt).reshape(k, d2, r), s
+    return (u * s[None, :]).reshape(l, d1, k), vt.reshape(k, d2, r), s
+
+
+@dataclass
+class OracleDMRG2(OracleDMRG):
+    """Two-site finite DMRG with SVD truncation to ``chi``. The reference's
+    two-site solver is a stub (``finite_dmrg.py:113-114``); this is the
+    textbook algorithm on the same environments, eigensolver switch and
+    convergence rule as the one-site driver."""
+
+    chi: int = 0
+
+    def solve2(self, site: int, tol: float):
+        A, B = self.mps[site], self.mps[site + 1]
+        theta = np.tensordot(A, B, axes=(2, 0))
+        shape = theta.shape
+        v0 = theta.reshape(-1)
+        L, R = self.left[site], self.right[site + 1]
+        W1, W2 = self.mpo[site], self.mpo[site + 1]
+        if v0.size < self.exact_solver_dim:
+            M = np.einsum("awb,wukc,uvld,rvs->aklrbcds", L, W1, W2, R).reshape(v0.size, v0.size)
+            w, V = np.linalg.eigh(M)
+            return float(w[0]), V[:, 0].reshape(shape)
+        N = v0.size
+
+        def mv(x):
+            self.matvecs += 1
+            return two_site_matvec(L, W1, W2, R, np.ravel(x).reshape(shape))
+
+        op = spla.LinearOperator((N, N), matvec=mv, dtype=float)
+        w, V = spla.eigsh(op, k=1, which="SA", v0=v0, tol=tol)
+        return float(w[0]), V[:, 0].reshape(shape)
+
+    def sweep(self, direction: int, tol: float) -> float:
+        sites = range(self.n - 1) if direction == RIGHTWARD else range(self.n - 2, -1, -1)
+        energy = None
+        for site in sites:
+            energy, theta = self.solve2(site, tol)
+            self.site_energies.append(energy)
+            A, B, _ = split_two_site_svd(theta, self.chi, direction)
+            self.mps[site], self.mps[site + 1] = A, B
+            if direction == RIGHTWARD:
+                self.update_left(site + 1)
+            else:
+                self.update_right(site)
+        return energy
+
+
 def xx_chain_ground_energy(n: int) -> float:
     """Closed form for XXZ at delta=0 (free fermions, open chain):
     E0 = sum of negative single-particle energies -cos(k pi / (n+1))."""

@@ -357,3 +357,23 @@ def test_eigsh_clustered(gpu, prec, max_basis):
     assert abs(info.eval - w[0]) <= 1e-12 * nrm
     v = host(vec)
     assert np.linalg.norm(M @ v - w[0] * v) <= 2e-10 * nrm
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", [(1, 1, 5, 5, 2, 2), (4, 5, 5, 5, 2, 8), (16, 5, 5, 1, 2, 1),
+                                   (12, 5, 5, 5, 2, 9), (7, 4, 3, 5, 3, 6), (32, 5, 5, 5, 2, 32)])
+def test_heff2_matvec(gpu, prec, shape):
+    chil, Dl, Dm, Dr, d, chir = shape
+    print(f"heff2 {prec} {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape) + 7)
+    Lt = rng.standard_normal((chil, Dl, chil))
+    W1 = rng.standard_normal((Dl, Dm, d, d))
+    W2 = rng.standard_normal((Dm, Dr, d, d))
+    Rt = rng.standard_normal((chir, Dr, chir))
+    x = rng.standard_normal((chil, d, d, chir))
+    y = torch.empty(x.size, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_heff2_workspace_bytes(*shape))
+    check(L().tnx_heff2_matvec(dev(Lt).data_ptr(), dev(W1).data_ptr(), dev(W2).data_ptr(), dev(Rt).data_ptr(),
+                               dev(x).data_ptr(), y.data_ptr(), *shape, PREC[prec], wp, wb, stream()), "heff2")
+    ref = o.two_site_matvec(Lt, W1, W2, Rt, x)
+    assert relerr(host(y), ref) < 10 * TOL[prec]

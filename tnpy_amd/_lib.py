@@ -70,6 +70,13 @@ SIGNATURES = {
         _i,
         [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, C.POINTER(EigshOpts), C.POINTER(EigshInfo), _p, _sz, _p],
     ),
+    "tnx_heff2_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "tnx_heff2_matvec": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_eigsh_heff2_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
+    "tnx_eigsh_heff2": (
+        _i,
+        [_p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, C.POINTER(EigshOpts), C.POINTER(EigshInfo), _p, _sz, _p],
+    ),
     "tnx_gauge_workspace_bytes": (_sz, [_i, _i, _i]),
     "tnx_gauge_right": (_i, [_p, _p, _p, _i, _i, _i, _i, _p, _sz, _p]),
     "tnx_gauge_left": (_i, [_p, _p, _p, _i, _i, _i, _i, _p, _sz, _p]),

@@ -142,6 +142,68 @@ int heff1_matvec(const double* L, const double* W, const double* R,
   return gemm(g3, s);
 }
 
+// Two-site effective Hamiltonian (no reference function: the two-site
+// solver is a stub at tnpy/finite_dmrg.py:113-114):
+//   y[l',c1,c2,r'] = sum L[l,w,l'] W1[w,u,k1,c1] W2[u,v,k2,c2] R[r,v,r'] x[l,k1,k2,r]
+//   T1[(l,k1,k2),(v,r')] = X[(l,k1,k2), r] R[r,(v,r')]          GEMM, K = chir
+//   T2[l,k1,u,c2,r']     = sum_k2,v W2[u,v,k2,c2] T1[l,k1,k2,v,r']   W-apply
+//   T3[l,w,c1,c2,r']     = sum_k1,u W1[w,u,k1,c1] T2[l,k1,u,c2,r']   W-apply
+//   y[l',(c1,c2,r')]     = sum_(l,w) L[(l,w),l'] T3[(l,w),(c1,c2,r')]  GEMM, K = chil*Dl
+// T3 reuses T1's memory.
+size_t heff2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir) {
+  const size_t t1 = (size_t)chil * d * d * Dr * chir;
+  const size_t t3 = (size_t)chil * Dl * d * d * chir;
+  const size_t t2 = (size_t)chil * d * Dm * d * chir;
+  return (std::max(t1, t3) + t2) * 8 + 256;
+}
+
+int heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
+                 const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d, int chir,
+                 int prec, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dm > 0 && Dr > 0 && d > 0 && chir > 0, "heff2_matvec: bad dims");
+  TNX_CHECK_ARG(L && W1 && W2 && R && x && y, "heff2_matvec: null pointer");
+  TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "heff2_matvec: bad precision");
+  TNX_CHECK_ARG(ws && ws_bytes >= heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir),
+                "heff2_matvec: workspace too small");
+  const size_t eb = inter_bytes(prec);
+  const DT it = prec == TNX_F64 ? DT::F64 : DT::F32;
+  const size_t t1 = (size_t)chil * d * d * Dr * chir;
+  const size_t t3 = (size_t)chil * Dl * d * d * chir;
+  char* T1 = (char*)ws;
+  char* T2 = T1 + ((std::max(t1, t3) * eb + 255) & ~(size_t)255);
+  char* T3 = T1;
+  GemmDesc g1{};
+  g1.M = chil * d * d; g1.N = Dr * chir; g1.K = chir; g1.batch = 1;
+  g1.A = x; g1.ta = DT::F64; g1.lda = chir; g1.sA = 0; g1.a_kcontig = true;
+  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = 0; g1.b_ncontig = true;
+  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = 0;
+  g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
+  TNX_TRY(gemm(g1, s));
+  WApply a2{};  // W2 over (k2, v) -> (u, c2), outer (l,k1), inner r'
+  a2.in = T1; a2.out = T2; a2.W = W2; a2.in_f32 = a2.out_f32 = (prec != TNX_F64);
+  a2.n_outer = (int64_t)chil * d; a2.n_inner = chir;
+  a2.in_so = (int64_t)d * Dr * chir; a2.in_sx = (int64_t)Dr * chir; a2.in_sy = chir;
+  a2.out_so = (int64_t)Dm * d * chir; a2.out_sp = (int64_t)d * chir; a2.out_sq = chir;
+  a2.NX = d; a2.NY = Dr; a2.NP = Dm; a2.NQ = d;
+  a2.Dl = Dm; a2.Dr = Dr; a2.d = d; a2.mode = 0;
+  TNX_TRY(launch_wapply(a2, s));
+  WApply a1{};  // W1 over (k1, u) -> (w, c1), outer l, inner (c2, r')
+  a1.in = T2; a1.out = T3; a1.W = W1; a1.in_f32 = a1.out_f32 = (prec != TNX_F64);
+  a1.n_outer = chil; a1.n_inner = (int64_t)d * chir;
+  a1.in_so = (int64_t)d * Dm * d * chir; a1.in_sx = (int64_t)Dm * d * chir; a1.in_sy = (int64_t)d * chir;
+  a1.out_so = (int64_t)Dl * d * d * chir; a1.out_sp = (int64_t)d * d * chir; a1.out_sq = (int64_t)d * chir;
+  a1.NX = d; a1.NY = Dm; a1.NP = Dl; a1.NQ = d;
+  a1.Dl = Dl; a1.Dr = Dm; a1.d = d; a1.mode = 0;
+  TNX_TRY(launch_wapply(a1, s));
+  GemmDesc g3{};
+  g3.M = chil; g3.N = d * d * chir; g3.K = chil * Dl; g3.batch = 1;
+  g3.A = L; g3.ta = DT::F64; g3.lda = chil; g3.sA = 0; g3.a_kcontig = false;
+  g3.B = T3; g3.tb = it; g3.ldb = (int64_t)d * d * chir; g3.sB = 0; g3.b_ncontig = true;
+  g3.C = y; g3.tc = DT::F64; g3.ldc = (int64_t)d * d * chir; g3.sC = 0;
+  g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
+  return gemm(g3, s);
+}
+
 size_t env_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
   return (size_t)chil * d * chir * (Dl + Dr) * 8 + 256;
 }
@@ -328,6 +390,17 @@ int tnx_heff1_dense(const double* L, const double* W, const double* R,
 
 size_t tnx_env_workspace_bytes(int chil, int Dl, int Dr, int d, int chir) {
   return tnx::env_ws_bytes(chil, Dl, Dr, d, chir);
+}
+
+size_t tnx_heff2_workspace_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir) {
+  return tnx::heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir);
+}
+
+int tnx_heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
+                     const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d,
+                     int chir, int precision, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::heff2_matvec(L, W1, W2, R, x, y, chil, Dl, Dm, Dr, d, chir, precision, ws,
+                           ws_bytes, (hipStream_t)stream);
 }
 
 int tnx_env_left(const double* L, const double* A, const double* W,

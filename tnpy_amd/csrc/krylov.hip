@@ -31,6 +31,10 @@ int heff1_matvec(const double* L, const double* W, const double* R, const double
                  double* y, int chil, int Dl, int Dr, int d, int chir, int prec,
                  void* ws, size_t ws_bytes, hipStream_t s);
 size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir);
+int heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
+                 const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d, int chir,
+                 int prec, void* ws, size_t ws_bytes, hipStream_t s);
+size_t heff2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir);
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -600,14 +604,13 @@ static int keep_of(int m, int keep) {
   return std::max(1, std::min(k, m - 2));
 }
 
-size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
+static size_t eigsh_ws_bytes_n(size_t N, size_t mvb, int max_basis) {
   const int m = std::max(3, std::min(max_basis, kMaxBasis));
-  const size_t N = (size_t)chil * d * chir;
   size_t b = 0;
   b += align256((size_t)(m + 1) * N * 8);
   b += align256((size_t)(m - 2) * N * 8);
   b += 2 * align256(N * 8);
-  b += align256(heff1_ws_bytes(chil, Dl, Dr, d, chir));
+  b += align256(mvb);
   b += align256((size_t)kCgsBlocks * kPartStride * 8);
   b += 3 * align256(kPartStride * 8);
   b += 3 * align256((kMaxBasis + 2) * 8);
@@ -615,6 +618,15 @@ size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) 
   b += align256((kMaxBasis + 2) * 8);
   b += 2 * align256(16 * 8);
   return b + 256;
+}
+
+size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
+  return eigsh_ws_bytes_n((size_t)chil * d * chir, heff1_ws_bytes(chil, Dl, Dr, d, chir), max_basis);
+}
+
+size_t eigsh2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir, int max_basis) {
+  return eigsh_ws_bytes_n((size_t)chil * d * d * chir, heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir),
+                          max_basis);
 }
 
 static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
@@ -761,9 +773,12 @@ __global__ void outer_scalars_kernel(const double* h1, const double* h2, const d
   }
 }
 
+// The operator of one eigensolve: the one-site H_eff (W2 == nullptr) or
+// the two-site H_eff (W, W2 with middle MPO bond Dm).
 struct EigCtx {
   const double *L, *W, *R;
-  int chil, Dl, Dr, d, chir;
+  const double* W2 = nullptr;
+  int chil, Dl, Dm = 0, Dr, d, chir;
   int64_t N;
   hipStream_t s;
   KWork k;
@@ -773,6 +788,9 @@ struct EigCtx {
 static int apply_h(EigCtx& c, const double* x, double* y, int prec) {
   c.matvecs++;
   if (prec == TNX_F64) c.matvecs64++;
+  if (c.W2)
+    return heff2_matvec(c.L, c.W, c.W2, c.R, x, y, c.chil, c.Dl, c.Dm, c.Dr, c.d, c.chir, prec,
+                        c.k.mv, c.k.mv_bytes, c.s);
   return heff1_matvec(c.L, c.W, c.R, x, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, c.k.mv,
                       c.k.mv_bytes, c.s);
 }
@@ -803,6 +821,10 @@ static int outer_step(EigCtx& c, double* theta, double* rho) {
   return TNX_OK;
 }
 
+static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
+                     const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                     size_t ws_bytes, size_t mvb);
+
 int eigsh_heff1(const double* L, const double* W, const double* R, const double* v0,
                 double* vec, double* hvec, int chil, int Dl, int Dr, int d, int chir,
                 const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
@@ -810,16 +832,39 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
   TNX_CHECK_ARG(opts && info, "eigsh: null opts/info");
   TNX_CHECK_ARG(L && W && R && v0 && vec, "eigsh: null pointer");
   TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dr > 0 && d > 0 && chir > 0, "eigsh: bad dims");
-  TNX_CHECK_ARG(opts->precision >= TNX_F64 && opts->precision <= TNX_BF16X3, "eigsh: bad precision");
-  TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh: negative tol");
-  const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
-  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_ws_bytes(chil, Dl, Dr, d, chir, m), "eigsh: workspace too small");
   EigCtx c;
   c.L = L; c.W = W; c.R = R;
   c.chil = chil; c.Dl = Dl; c.Dr = Dr; c.d = d; c.chir = chir;
   c.N = (int64_t)chil * d * chir;
   c.s = s;
-  carve(c.k, ws, m, (size_t)c.N, heff1_ws_bytes(chil, Dl, Dr, d, chir));
+  return eigsh_run(c, v0, vec, hvec, opts, info, ws, ws_bytes, heff1_ws_bytes(chil, Dl, Dr, d, chir));
+}
+
+int eigsh_heff2(const double* L, const double* W1, const double* W2, const double* R,
+                const double* v0, double* vec, double* hvec, int chil, int Dl, int Dm, int Dr,
+                int d, int chir, const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(opts && info, "eigsh2: null opts/info");
+  TNX_CHECK_ARG(L && W1 && W2 && R && v0 && vec, "eigsh2: null pointer");
+  TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dm > 0 && Dr > 0 && d > 0 && chir > 0, "eigsh2: bad dims");
+  EigCtx c;
+  c.L = L; c.W = W1; c.W2 = W2; c.R = R;
+  c.chil = chil; c.Dl = Dl; c.Dm = Dm; c.Dr = Dr; c.d = d; c.chir = chir;
+  c.N = (int64_t)chil * d * d * chir;
+  c.s = s;
+  return eigsh_run(c, v0, vec, hvec, opts, info, ws, ws_bytes,
+                   heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir));
+}
+
+static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
+                     const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                     size_t ws_bytes, size_t mvb) {
+  TNX_CHECK_ARG(opts->precision >= TNX_F64 && opts->precision <= TNX_BF16X3, "eigsh: bad precision");
+  TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh: negative tol");
+  const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
+  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_ws_bytes_n((size_t)c.N, mvb, m), "eigsh: workspace too small");
+  hipStream_t s = c.s;
+  carve(c.k, ws, m, (size_t)c.N, mvb);
   KWork& k = c.k;
   const int64_t N = c.N;
   const int prec = opts->precision;
@@ -845,6 +890,7 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
   // spectrum, e.g. clusters narrower than its rounding).
   int cprec = prec;
   double rho_prev = 0.0;
+  bool first_cycle = true;
 
   while (true) {
     TNX_TRY(outer_step(c, &theta, &rho));
@@ -863,9 +909,17 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
     hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, k.V + N, N);
     TNX_LAUNCH_CHECK();
     // Inner Lanczos cycle in `prec`, from column j = 1.
+    // Inner target. FP64: the final tolerance. Low precision: the first
+    // cycle runs the full thick-restart Lanczos down to the rounding floor of
+    // its arithmetic (it keeps its restart subspace the whole way); later
+    // cycles start from [y, r/|r|] and only need to cut the FP64 residual.
     const double rho_outer = rho;
-    const double inner_tol = mixed ? std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer))
-                                   : tol * anorm;
+    const double unit_lp = cprec == TNX_F32 ? 5.96e-8 : 3.9e-6;
+    const double inner_tol =
+        !mixed ? tol * anorm
+               : (first_cycle ? std::max(tol, 4.0 * unit_lp) * anorm
+                              : std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer)));
+    first_cycle = false;
     int j = 1;
     bool inner_done = false;
     have_hy = false;  // the inner cycle reuses k.w
@@ -987,6 +1041,19 @@ int tnx_syev_lowest(const double* M, int n, double* evec, double* eval_host, voi
 size_t tnx_eigsh_heff1_workspace_bytes(int chil, int Dl, int Dr, int d, int chir,
                                        int max_basis) {
   return tnx::eigsh_ws_bytes(chil, Dl, Dr, d, chir, max_basis);
+}
+
+size_t tnx_eigsh_heff2_workspace_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir,
+                                       int max_basis) {
+  return tnx::eigsh2_ws_bytes(chil, Dl, Dm, Dr, d, chir, max_basis);
+}
+
+int tnx_eigsh_heff2(const double* L, const double* W1, const double* W2, const double* R,
+                    const double* v0, double* vec, double* hvec, int chil, int Dl, int Dm,
+                    int Dr, int d, int chir, const tnx_eigsh_opts_t* opts,
+                    tnx_eigsh_info_t* info, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::eigsh_heff2(L, W1, W2, R, v0, vec, hvec, chil, Dl, Dm, Dr, d, chir, opts, info,
+                          ws, ws_bytes, (hipStream_t)stream);
 }
 
 int tnx_eigsh_heff1(const double* L, const double* W, const double* R, const double* v0,

@@ -53,7 +53,10 @@ class FiniteDMRG:
             list of W arrays, or a quimb MPO).
         bond_dim: bond dimension of the random initial MPS (``chi`` is an
             alias, as in the reference README).
-        block_size: accepted for signature compatibility; one-site only.
+        block_size: sites per local update: 1 (one-site DMRG with the
+            1e-5 perturbation, the reference's algorithm) or 2 (two-site
+            DMRG with truncation to ``bond_dim``; the reference stores this
+            argument but its two-site solver is a stub).
         mps: optional initial state (``MatrixProductState`` or arrays).
         exact_solver_dim: local dimension below which the dense eigensolver
             is used (``finite_dmrg.py:108-110``).
@@ -93,6 +96,8 @@ class FiniteDMRG:
             raise TypeError("FiniteDMRG needs a bond dimension (bond_dim or chi)")
         if precision not in _lib.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+        if block_size not in (1, 2):
+            raise ValueError("block_size must be 1 (one-site) or 2 (two-site)")
         self.device = device if device is not None else _lib.require_gpu()
         _lib.load_library()
         mpo = as_mpo(mpo)
@@ -215,8 +220,96 @@ class FiniteDMRG:
         e, vec, _ = self._solve(site, tol, **kwargs)
         return e, vec.cpu().numpy().reshape(-1, 1)
 
-    def two_site_solver(self, site: int, tol: float = 1e-8, **kwargs):
-        return NotImplemented
+    # -- two-site update (the reference's two_site_solver is a stub,
+    #    finite_dmrg.py:113-114; block_size=2 selects this path) ------------
+    def _gemm(self, M, N, K, A, lda, ak, B, ldb, bn, C, ldc):
+        _lib.check(
+            _lib.lib().tnx_gemm_f64(M, N, K, 1, 1.0, _lib.ptr(A, "A"), lda, 0, int(ak), _lib.ptr(B, "B"), ldb, 0,
+                                    int(bn), 0.0, _lib.ptr(C, "C"), ldc, 0, _lib.TNX_F64,
+                                    _lib.stream_ptr(self.device)),
+            "tnx_gemm_f64",
+        )
+
+    def _solve2(self, site: int, tol: float, **kwargs) -> Tuple[float, torch.Tensor]:
+        env = self._env
+        A, B = env.mps.tensors[site], env.mps.tensors[site + 1]
+        l, d, mid = A.shape
+        _, d2, r = B.shape
+        theta = torch.empty(l, d, d2, r, dtype=torch.float64, device=self.device)
+        self._gemm(l * d, d2 * r, mid, A, mid, True, B, d2 * r, True, theta, d2 * r)
+        if site > 0 and site not in env.left:
+            env.ensure_left(site)
+        W1, W2 = env._W[site], env._W[site + 1]
+        Dl, Dm, Dr = int(W1.shape[0]), int(W1.shape[1]), int(W2.shape[1])
+        N = theta.numel()
+        vec = torch.empty(N, dtype=torch.float64, device=self.device)
+        hvec = torch.empty(N, dtype=torch.float64, device=self.device)
+        lib = _lib.lib()
+        opts = self._opts(tol, kwargs)
+        info = _lib.EigshInfo()
+        wp, wb = env.workspace.get(lib.tnx_eigsh_heff2_workspace_bytes(l, Dl, Dm, Dr, d, r, opts.max_basis))
+        st = lib.tnx_eigsh_heff2(
+            _lib.ptr(env.left[site], "L"), _lib.ptr(W1, "W1"), _lib.ptr(W2, "W2"),
+            _lib.ptr(env.right[site + 1], "R"), _lib.ptr(theta.reshape(-1), "v0"), _lib.ptr(vec, "vec"),
+            _lib.ptr(hvec, "hvec"), l, Dl, Dm, Dr, d, r, C.byref(opts), C.byref(info), wp, wb,
+            _lib.stream_ptr(self.device),
+        )
+        if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
+            _lib.check(st, "tnx_eigsh_heff2")
+        self.stats["matvecs"] += info.matvecs
+        self.stats["matvecs_f64"] += info.matvecs_f64
+        self.stats["restarts"] += info.restarts
+        self.stats["krylov_sites"] += 1
+        return float(info.eval), vec.reshape(l, d, d2, r)
+
+    def two_site_solver(self, site: int, tol: float = 1e-8, **kwargs) -> Tuple[float, np.ndarray]:
+        """Lowest eigenpair of the two-site H_eff on sites (site, site+1);
+        returns the energy and the unit two-site tensor (l, d, d, r)."""
+        e, theta = self._solve2(site, tol, **kwargs)
+        return e, theta.cpu().numpy()
+
+    def _split2(self, site: int, theta: torch.Tensor, direction: Direction) -> torch.Tensor:
+        """Truncated split of theta into two sites keeping
+        min(chi, l*d, d*r) states: the dominant eigenvectors of the reduced
+        density matrix theta theta^T (rightward: site <- U, next <- U^T theta)
+        or theta^T theta (leftward: next <- V^T, site <- theta V). Returns the
+        kept weights (squared singular values, descending)."""
+        l, d1, d2, r = theta.shape
+        m, n = l * d1, d2 * r
+        k = min(self.bond_dim, m, n)
+        M = theta.reshape(m, n)
+        if direction == Direction.RIGHTWARD:
+            rho = torch.empty(m, m, dtype=torch.float64, device=self.device)
+            self._gemm(m, m, n, M, n, True, M, n, False, rho, m)
+            w, U = torch.linalg.eigh(rho)
+            U = U[:, m - k:].flip(-1).contiguous()
+            Bn = torch.empty(k, n, dtype=torch.float64, device=self.device)
+            self._gemm(k, n, m, U, k, False, M, n, True, Bn, n)
+            self.mps._tensors[site] = U.reshape(l, d1, k)
+            self.mps._tensors[site + 1] = Bn.reshape(k, d2, r)
+        else:
+            rho = torch.empty(n, n, dtype=torch.float64, device=self.device)
+            self._gemm(n, n, m, M, n, False, M, n, True, rho, n)
+            w, V = torch.linalg.eigh(rho)
+            V = V[:, n - k:].flip(-1).contiguous()
+            An = torch.empty(m, k, dtype=torch.float64, device=self.device)
+            self._gemm(m, k, n, M, n, True, V, k, True, An, k)
+            self.mps._tensors[site] = An.reshape(l, d1, k)
+            self.mps._tensors[site + 1] = V.T.contiguous().reshape(k, d2, r)
+        return w.flip(-1)[:k]
+
+    def _two_site_step(self, site: int, direction: Direction, tol: float, **kwargs) -> float:
+        energy, theta = self._solve2(site, tol, **kwargs)
+        logger.info(f"Sweeping to sites [{site + 1},{site + 2}/{self.n_sites}], E0 = {energy}")
+        self._split2(site, theta, direction)
+        if direction == Direction.RIGHTWARD:
+            self._env.update_left(site + 1)
+        else:
+            self._env.update_right(site)
+        self.site_energies.append(energy)
+        if self.on_site is not None:
+            self.on_site(site, energy)
+        return energy
 
     def perturb_wave_function(self, site: int, alpha: float = 1e-5) -> None:
         """psi <- psi + alpha H_eff psi in place, not renormalised
@@ -251,6 +344,14 @@ class FiniteDMRG:
     def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8, **kwargs) -> Optional[float]:
         """One directional sweep (``finite_dmrg.py:143-171``); returns the
         energy at its last site."""
+        if self._block_size == 2:
+            if direction == Direction.LEFTWARD:
+                self._env.ensure_left(self.n_sites - 2)
+            sites = range(self.n_sites - 1) if direction == Direction.RIGHTWARD else range(self.n_sites - 2, -1, -1)
+            energy = None
+            for site in sites:
+                energy = self._two_site_step(site, direction, tol, **kwargs)
+            return energy
         if direction == Direction.LEFTWARD:
             self._env.ensure_left(self.n_sites - 1)
         iter_sites = (
