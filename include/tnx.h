@@ -61,6 +61,14 @@ int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
                  double beta, double* C, int64_t ldc, int64_t strideC,
                  int precision, tnx_stream_t stream);
 
+/* Runtime tile autotuner of the GEMMs (on by default; TNX_AUTOTUNE=0 in the
+ * environment turns it off). The first GEMM of each shape times every tile
+ * configuration on a scratch output and caches the fastest for the process.
+ * The choice never changes results: only split-K changes the summation
+ * order, and it follows a fixed rule on the shape. on=0 also clears the
+ * cache. No reference counterpart (quimb leaves this to BLAS). */
+int tnx_set_autotune(int on);
+
 /* ---- one-site effective Hamiltonian matvec -----------------------------
  * y[l'][b][r'] = sum L[l][wl][l'] W[wl][wr][k][b] R[r][wr][r'] x[l][k][r]
  * Replaces Environment.one_site_matvec's closure

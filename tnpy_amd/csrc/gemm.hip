@@ -19,6 +19,11 @@
 // MFMAs of tile t; LDS is double-buffered (one barrier per k-tile).
 #include "tnx_common.h"
 
+#include <cmath>
+#include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+
 #ifndef TNX_BK64
 #define TNX_BK64 32
 #endif
@@ -53,16 +58,17 @@ template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; using accs
 template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; using accs_t = float;  static constexpr int BK = TNX_BK32; };
 template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 32; };
 
-template <int PREC, int BM, int BN, bool A_K, bool B_N>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+// WGM x WGN wavefronts per block; each wave owns a (BM/WGM) x (BN/WGN) tile.
+template <int PREC, int BM, int BN, bool A_K, bool B_N, int WGM, int WGN, int BK>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   using lds_t = typename PT<PREC>::lds_t;
   using acc_t = typename PT<PREC>::acc_t;
-  constexpr int BK = PT<PREC>::BK;
-  constexpr int NT = 256;
+  constexpr int NT = 64 * WGM * WGN;
   constexpr int EA = BM * BK / NT;
   constexpr int EB = BK * BN / NT;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
+  static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile not divisible by the block");
   // LDS geometry. F64/F32: [BK][BM + PADM]. BF16X3: [BM][BK + PADK] for
   // hi and lo parts (the k index is innermost so a lane's 8 values are
   // one 16-byte read).
@@ -77,8 +83,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm0 = (wave >> 1) * WM;
-  const int wn0 = (wave & 1) * WN;
+  const int wm0 = (wave / WGN) * WM;
+  const int wn0 = (wave % WGN) * WN;
 
   const int bat = blockIdx.y;
   const int tile = blockIdx.x;
@@ -293,59 +299,197 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, i
   }
 }
 
-template <int PREC, int BM, int BN>
-static int launch_tiles(const GemmDesc& d, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// Tile configurations, split-K rule and the runtime autotuner.
+//
+// A config only changes how output tiles are cut and scheduled: every output
+// element still accumulates its K range in order, so results are bitwise
+// identical whichever config runs. Split-K does change the rounding and is
+// therefore decided by a fixed rule on the shape, never by timing.
+// ---------------------------------------------------------------------------
+struct Cfg { int bm, bn, wgm, wgn, bk; };
+
+template <int PREC> struct Cfgs;
+template <> struct Cfgs<0> {
+  static constexpr int n = 6;
+  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 16},
+                               {128, 256, 2, 4, 16}, {256, 160, 4, 2, 16}, {160, 256, 2, 4, 16}};
+};
+template <> struct Cfgs<1> {
+  static constexpr int n = 6;
+  static constexpr Cfg c[n] = {{64, 64, 2, 2, 16}, {128, 128, 2, 2, 16}, {256, 128, 4, 2, 16},
+                               {128, 256, 2, 4, 16}, {256, 160, 4, 2, 16}, {160, 256, 2, 4, 16}};
+};
+template <> struct Cfgs<2> {
+  static constexpr int n = 6;
+  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 32},
+                               {128, 256, 2, 4, 32}, {256, 160, 4, 2, 32}, {160, 256, 2, 4, 32}};
+};
+constexpr Cfg Cfgs<0>::c[];
+constexpr Cfg Cfgs<1>::c[];
+constexpr Cfg Cfgs<2>::c[];
+
+template <int PREC, int BM, int BN, int WGM, int WGN, int BK>
+static void launch_cfg(const GemmDesc& d, const GemmArgs& g0, int splits, hipStream_t s) {
+  GemmArgs g = g0;
+  g.tiles_n = ceil_div(d.N, BN);
+  const int tiles = ceil_div(d.M, BM) * g.tiles_n;
+  dim3 grid(tiles, d.batch, splits);
+  dim3 block(64 * WGM * WGN);
+  if (d.a_kcontig && d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, true, WGM, WGN, BK>), grid, block, 0, s, g);
+  else if (d.a_kcontig && !d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, false, WGM, WGN, BK>), grid, block, 0, s, g);
+  else if (!d.a_kcontig && d.b_ncontig)
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, true, WGM, WGN, BK>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, false, WGM, WGN, BK>), grid, block, 0, s, g);
+}
+
+template <int PREC>
+static void launch_index(int ci, const GemmDesc& d, const GemmArgs& g, int splits, hipStream_t s) {
+#define TNX_CFG(I)                                                                              \
+  if (ci == I && I < Cfgs<PREC>::n) {                                                           \
+    constexpr Cfg c = Cfgs<PREC>::c[I < Cfgs<PREC>::n ? I : 0];                                 \
+    launch_cfg<PREC, c.bm, c.bn, c.wgm, c.wgn, c.bk>(d, g, splits, s);                          \
+    return;                                                                                     \
+  }
+  TNX_CFG(0) TNX_CFG(1) TNX_CFG(2) TNX_CFG(3) TNX_CFG(4) TNX_CFG(5) TNX_CFG(6)
+#undef TNX_CFG
+}
+
+// Deterministic split-K rule: split only when the output is small (fewer
+// than ~256 tiles of 128x128) and K is long, and the scratch fits.
+static int split_rule(const GemmDesc& d) {
+  if (!d.splitk_ws) return 1;
+  const int64_t elems = (int64_t)d.M * d.N * d.batch;
+  const int64_t target = (int64_t)256 * 128 * 128;
+  if (elems >= target || d.K < 1024) return 1;
+  int sp = 1;
+  while (sp < 32 && elems * sp * 2 <= target && (int64_t)d.K / (sp * 2) >= 256) sp *= 2;
+  while (sp > 1 && (size_t)sp * elems * sizeof(double) > d.splitk_ws_bytes) sp /= 2;
+  return sp;
+}
+
+// Heuristic when not autotuning: fewest full-chip waves, weighted by the
+// per-CU efficiency of the tile size (larger tiles reuse operands more).
+template <int PREC>
+static int heuristic_cfg(const GemmDesc& d, int splits) {
+  int best = 0;
+  double best_t = 1e300;
+  for (int i = 0; i < Cfgs<PREC>::n; ++i) {
+    const Cfg& c = Cfgs<PREC>::c[i];
+    const int64_t blocks = (int64_t)ceil_div(d.M, c.bm) * ceil_div(d.N, c.bn) * d.batch * splits;
+    const int per_cu = (c.wgm * c.wgn <= 4) ? 2 : 1;
+    const double waves = std::ceil((double)blocks / (256.0 * per_cu));
+    const double eff = (c.bm * c.bn >= 256 * 128) ? 0.8 : (c.bm * c.bn >= 128 * 128 ? 0.6 : 0.45);
+    const double t = waves * (double)c.bm * c.bn / (per_cu * eff);
+    if (t < best_t - 1e-9) { best_t = t; best = i; }
+  }
+  return best;
+}
+
+struct TuneKey {
+  int prec, M, N, K, batch, ak, bn, ta, tb, tc, splits;
+  bool operator==(const TuneKey& o) const {
+    return prec == o.prec && M == o.M && N == o.N && K == o.K && batch == o.batch && ak == o.ak &&
+           bn == o.bn && ta == o.ta && tb == o.tb && tc == o.tc && splits == o.splits;
+  }
+};
+struct TuneHash {
+  size_t operator()(const TuneKey& k) const {
+    size_t h = 1469598103934665603ull;
+    for (int v : {k.prec, k.M, k.N, k.K, k.batch, k.ak, k.bn, k.ta, k.tb, k.tc, k.splits})
+      h = (h ^ (size_t)(unsigned)v) * 1099511628211ull;
+    return h;
+  }
+};
+
+static std::mutex g_tune_mu;
+static std::unordered_map<TuneKey, int, TuneHash> g_tune;
+static int g_autotune = -1;  // -1: from TNX_AUTOTUNE (default on)
+
+static bool autotune_on() {
+  if (g_autotune < 0) {
+    const char* e = getenv("TNX_AUTOTUNE");
+    g_autotune = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_autotune == 1;
+}
+
+template <int PREC>
+static int run_prec(const GemmDesc& d, hipStream_t s) {
   GemmArgs g;
   g.M = d.M; g.N = d.N; g.K = d.K;
   g.A = d.A; g.lda = d.lda; g.sA = d.sA; g.ta = (int)d.ta;
   g.B = d.B; g.ldb = d.ldb; g.sB = d.sB; g.tb = (int)d.tb;
   g.C = d.C; g.ldc = d.ldc; g.sC = d.sC; g.tc = (int)d.tc;
   g.alpha = d.alpha; g.beta = d.beta;
-  g.tiles_n = ceil_div(d.N, BN);
-  const int tiles = ceil_div(d.M, BM) * g.tiles_n;
-  // Split K when the output tiles cannot fill the chip and K is long.
-  int splits = 1;
+  g.tiles_n = 1;
+  const int splits = split_rule(d);
   g.kc = d.K;
   g.partial = nullptr;
-  const int64_t blocks = (int64_t)tiles * d.batch;
-  if (d.splitk_ws && blocks < 128 && d.K >= 512) {
-    constexpr int BK = PT<PREC>::BK;
-    int want = (int)std::min<int64_t>(256 / std::max<int64_t>(blocks, 1), d.K / 256);
-    want = std::max(1, std::min(want, 32));
-    const size_t need = (size_t)want * d.batch * d.M * d.N * sizeof(double);
-    if (want > 1 && need <= d.splitk_ws_bytes) {
-      splits = want;
-      g.kc = ((d.K + splits - 1) / splits + BK - 1) / BK * BK;
-      splits = ceil_div(d.K, g.kc);
-      g.partial = d.splitk_ws;
-    }
+  if (splits > 1) {
+    g.kc = ((d.K + splits - 1) / splits + 31) / 32 * 32;
+    g.partial = d.splitk_ws;
   }
-  dim3 grid(tiles, d.batch, splits);
-  dim3 block(256);
-  if (d.a_kcontig && d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, true>), grid, block, 0, s, g);
-  else if (d.a_kcontig && !d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, false>), grid, block, 0, s, g);
-  else if (!d.a_kcontig && d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, true>), grid, block, 0, s, g);
-  else
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, false>), grid, block, 0, s, g);
+  const int nsplit = splits > 1 ? ceil_div(d.K, g.kc) : 1;
+  const TuneKey key{PREC, d.M, d.N, d.K, d.batch, d.a_kcontig, d.b_ncontig, (int)d.ta, (int)d.tb,
+                    (int)d.tc, nsplit};
+  int ci = -1;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) ci = it->second;
+  }
+  if (ci < 0) {
+    ci = heuristic_cfg<PREC>(d, nsplit);
+    const int64_t work = (int64_t)d.M * d.N * d.K * d.batch;
+    if (autotune_on() && work >= ((int64_t)1 << 28)) {
+      // Time every config once on a scratch output (the caller's C is not
+      // touched), keep the fastest. Runs once per shape key.
+      GemmArgs gt = g;
+      void* scratch = nullptr;
+      const size_t cbytes = ((size_t)(d.batch - 1) * d.sC + (size_t)(d.M - 1) * d.ldc + d.N) *
+                            (d.tc == DT::F64 ? 8 : 4);
+      if (!g.partial && hipMalloc(&scratch, cbytes) == hipSuccess) {
+        gt.C = scratch;
+        gt.beta = 0.0;
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        float best = 1e30f;
+        for (int i = 0; i < Cfgs<PREC>::n; ++i) {
+          launch_index<PREC>(i, d, gt, nsplit, s);  // warm
+          hipEventRecord(e0, s);
+          launch_index<PREC>(i, d, gt, nsplit, s);
+          launch_index<PREC>(i, d, gt, nsplit, s);
+          hipEventRecord(e1, s);
+          hipEventSynchronize(e1);
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best) { best = ms; ci = i; }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipStreamSynchronize(s);
+        hipFree(scratch);
+        hipGetLastError();
+      }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = ci;
+  }
+  launch_index<PREC>(ci, d, g, nsplit, s);
   TNX_LAUNCH_CHECK();
   if (g.partial) {
     const int64_t MN = (int64_t)d.M * d.N;
     const int rb = (int)std::min<int64_t>((MN + 255) / 256, 2048);
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb, d.batch), dim3(256), 0, s, g.partial,
-                       splits, d.M, d.N, d.C, (int)d.tc, d.ldc, d.sC, d.alpha, d.beta);
+                       nsplit, d.M, d.N, d.C, (int)d.tc, d.ldc, d.sC, d.alpha, d.beta);
     TNX_LAUNCH_CHECK();
   }
   return TNX_OK;
-}
-
-template <int PREC>
-static int launch_prec(const GemmDesc& d, hipStream_t s) {
-  const int64_t big = (int64_t)ceil_div(d.M, 128) * ceil_div(d.N, 128) * d.batch;
-  if (big >= 200) return launch_tiles<PREC, 128, 128>(d, s);
-  return launch_tiles<PREC, 64, 64>(d, s);
 }
 
 int gemm(const GemmDesc& d, hipStream_t s) {
@@ -354,15 +498,26 @@ int gemm(const GemmDesc& d, hipStream_t s) {
   TNX_CHECK_ARG(d.A && d.B && d.C, "gemm: null operand");
   TNX_CHECK_ARG(d.batch <= 65535, "gemm: batch %d exceeds 65535", d.batch);
   switch (d.prec) {
-    case Prec::F64: return launch_prec<0>(d, s);
-    case Prec::F32: return launch_prec<1>(d, s);
-    case Prec::BF16X3: return launch_prec<2>(d, s);
+    case Prec::F64: return run_prec<0>(d, s);
+    case Prec::F32: return run_prec<1>(d, s);
+    case Prec::BF16X3: return run_prec<2>(d, s);
   }
   set_error("gemm: unknown precision");
   return TNX_ERR_INVALID;
 }
 
+void set_autotune(int on) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_autotune = on ? 1 : 0;
+  if (!on) g_tune.clear();
+}
+
 }  // namespace tnx
+
+extern "C" int tnx_set_autotune(int on) {
+  tnx::set_autotune(on);
+  return TNX_OK;
+}
 
 extern "C" int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
                             const double* A, int64_t lda, int64_t strideA, int a_kcontig,

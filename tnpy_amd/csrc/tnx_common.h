@@ -81,6 +81,7 @@ struct GemmDesc {
 };
 
 int gemm(const GemmDesc& g, hipStream_t stream);
+void set_autotune(int on);
 
 // Elementwise kernels used by several modules.
 int transpose_f64(const double* in, double* out, int rows, int cols,
