@@ -98,10 +98,15 @@ static bool valid_dims(int chil, int Dl, int Dr, int d, int chir) {
 // Element size of the intermediates for a given precision.
 static size_t inter_bytes(int prec) { return prec == TNX_F64 ? 8 : 4; }
 
+// Workspace: T1, T2, and FP64 split-K partials for the last GEMM (its
+// output, chil x d*chir, has few tiles at chil = chir and K = chil*Dl is
+// long; the split rule in gemm.hip decides how many are used).
+static size_t splitk_bytes(int64_t out_elems) { return (size_t)out_elems * 2 * 8; }
+
 size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
   const size_t t1 = (size_t)chil * d * Dr * chir;
   const size_t t2 = (size_t)chil * Dl * d * chir;
-  return (t1 + t2) * 8 + 256;
+  return (t1 + t2) * 8 + 512 + splitk_bytes((int64_t)chil * d * chir);
 }
 
 int heff1_matvec(const double* L, const double* W, const double* R,
@@ -116,6 +121,7 @@ int heff1_matvec(const double* L, const double* W, const double* R,
   const DT it = prec == TNX_F64 ? DT::F64 : DT::F32;
   char* T1 = (char*)ws;
   char* T2 = T1 + (((size_t)chil * d * Dr * chir * eb + 255) & ~(size_t)255);
+  char* P = T2 + (((size_t)chil * Dl * d * chir * eb + 255) & ~(size_t)255);
   // T1[(a,k),(v,s)] = X[(a,k), r] R[r, (v,s)]
   GemmDesc g1{};
   g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = 1;
@@ -139,6 +145,8 @@ int heff1_matvec(const double* L, const double* W, const double* R,
   g3.B = T2; g3.tb = it; g3.ldb = (int64_t)d * chir; g3.sB = 0; g3.b_ncontig = true;
   g3.C = y; g3.tc = DT::F64; g3.ldc = (int64_t)d * chir; g3.sC = 0;
   g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
+  g3.splitk_ws = reinterpret_cast<double*>(P);
+  g3.splitk_ws_bytes = splitk_bytes((int64_t)chil * d * chir);
   return gemm(g3, s);
 }
 

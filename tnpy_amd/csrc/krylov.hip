@@ -253,7 +253,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   const double eps = 2.220446049250313e-16;
   // 64-way multisection: each lane evaluates the Sturm count at one point.
   for (int it = 0; it < 12; ++it) {
-    if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
     const int c = sturm_count(a, b2, n, x, pivmin);
     const unsigned long long mask = __ballot(c >= 1);
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   if (lane == 0) {
     const double tiny = eps * fmax(tnorm, 1e-300);
     for (int i = 0; i < n; ++i) z[i] = 1.0 + 0.01 * (double)((i * 7919) % 13);
-    for (int rep = 0; rep < 3; ++rep) {
+    for (int rep = 0; rep < 2; ++rep) {
       // current pivot row: (c0, c1, c2) at columns i, i+1, i+2
       double c0 = a[0] - theta, c1 = b[0], c2 = 0.0, rc = z[0];
       for (int i = 0; i < n - 1; ++i) {
@@ -452,126 +452,84 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
 }
 
 // ---------------------------------------------------------------------------
-// Dense symmetric eigensolver for the projected matrix (n <= 64), one
-// workgroup: cyclic Jacobi with round-robin (circle) pair ordering, so the
-// n/2 rotations of a round act on disjoint planes and are applied together
-// (all row rotations, then all column rotations). Matrix and eigenvectors
-// stay in LDS. Writes eigenvalues ascending to w, the matching eigenvectors
-// as columns of S (row stride ldS), the lowest eigenvector to s, and the
-// Lanczos status: st[0] = lowest eigenvalue, st[1] = |bres * S[n-1][0]|,
-// st[2] = max |eigenvalue|.
+// All eigenpairs of a symmetric tridiagonal matrix (n <= 64): implicit QL
+// with Wilkinson shifts, one wavefront. Every lane runs the scalar
+// recurrence on the LDS copies of the diagonal / off-diagonal (identical
+// values); lane k owns row k of the eigenvector matrix Z and applies each
+// Givens rotation to it. Eigenvalues ascending to w, eigenvectors as columns
+// of S (row stride ldS). Used at thick restarts, where the projected matrix
+// is tridiagonal.
 // ---------------------------------------------------------------------------
-constexpr int kJac = 64;
-
-__global__ __launch_bounds__(256) void jacobi_eig_kernel(const double* __restrict__ Tin, int ldT,
-                                                         int n, double* __restrict__ w,
-                                                         double* __restrict__ S, int ldS,
-                                                         double* __restrict__ s,
-                                                         const double* __restrict__ bres,
-                                                         double* __restrict__ st) {
-  __shared__ double A[kJac][kJac + 1];
-  __shared__ double V[kJac][kJac + 1];
-  __shared__ double cs[kJac / 2], sn[kJac / 2];
-  __shared__ int P[kJac / 2], Q[kJac / 2];
-  __shared__ double red[4];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int m = n + (n & 1);  // even size; an odd n gets one decoupled dummy index
-  for (int e = tid; e < m * m; e += nt) {
-    const int i = e / m, j = e % m;
-    A[i][j] = (i < n && j < n) ? Tin[i * ldT + j] : 0.0;
-    V[i][j] = (i == j) ? 1.0 : 0.0;
+__global__ __launch_bounds__(64) void tridiag_all_kernel(const double* __restrict__ ad,
+                                                         const double* __restrict__ bd, int n,
+                                                         double* __restrict__ w,
+                                                         double* __restrict__ S, int ldS) {
+  __shared__ double d[64], e[64];
+  __shared__ double Z[64][65];
+  const int k = threadIdx.x;
+  if (k < n) {
+    d[k] = ad[k];
+    e[k] = (k < n - 1) ? bd[k] : 0.0;
+    for (int c = 0; c < n; ++c) Z[k][c] = (k == c) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const int half = m / 2;
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    // off-diagonal and total Frobenius norms
-    double off = 0.0, tot = 0.0;
-    for (int e = tid; e < m * m; e += nt) {
-      const int i = e / m, j = e % m;
-      const double a = A[i][j] * A[i][j];
-      tot += a;
-      if (i != j) off += a;
-    }
-    const int lane = tid & 63, wv = tid >> 6;
-    off = wave_sum(off);
-    tot = wave_sum(tot);
-    __syncthreads();
-    if (lane == 0) { red[wv] = off; }
-    __syncthreads();
-    double offs = 0.0;
-    for (int i = 0; i < (nt >> 6); ++i) offs += red[i];
-    __syncthreads();
-    if (lane == 0) { red[wv] = tot; }
-    __syncthreads();
-    double tots = 0.0;
-    for (int i = 0; i < (nt >> 6); ++i) tots += red[i];
-    if (offs <= 1e-32 * tots || m == 2 && sweep > 0) break;
-    for (int r = 0; r < m - 1; ++r) {
-      if (tid < half) {
-        auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + r) % (m - 1); };
-        int p = pos(tid), q = pos(m - 1 - tid);
-        if (p > q) { const int t = p; p = q; q = t; }
-        const double apq = A[p][q];
-        double c = 1.0, sgn = 0.0;
-        if (apq != 0.0) {
-          const double th = (A[q][q] - A[p][p]) / (2.0 * apq);
-          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-          c = 1.0 / sqrt(t * t + 1.0);
-          sgn = t * c;
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0; iter < 60; ++iter) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= eps * dd) break;
+      }
+      if (m == l) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double sn = 1.0, cs = 1.0, p = 0.0;
+      int i = m - 1;
+      bool early = false;
+      for (; i >= l; --i) {
+        double f = sn * e[i];
+        const double b = cs * e[i];
+        r = hypot(f, g);
+        __syncthreads();
+        if (k == 0) e[i + 1] = r;
+        __syncthreads();
+        if (r == 0.0) {
+          __syncthreads();
+          if (k == 0) { d[i + 1] -= p; e[m] = 0.0; }
+          __syncthreads();
+          early = true;
+          break;
         }
-        P[tid] = p; Q[tid] = q; cs[tid] = c; sn[tid] = sgn;
+        sn = f / r;
+        cs = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * cs * b;
+        p = sn * r;
+        __syncthreads();
+        if (k == 0) d[i + 1] = g + p;
+        __syncthreads();
+        g = cs * r - b;
+        if (k < n) {
+          f = Z[k][i + 1];
+          Z[k][i + 1] = sn * Z[k][i] + cs * f;
+          Z[k][i] = cs * Z[k][i] - sn * f;
+        }
       }
+      if (early) continue;
       __syncthreads();
-      for (int e = tid; e < half * m; e += nt) {  // rows: A <- J^T A
-        const int pr = e / m, k = e % m;
-        const int p = P[pr], q = Q[pr];
-        const double c = cs[pr], sg = sn[pr];
-        const double a = A[p][k], b = A[q][k];
-        A[p][k] = c * a - sg * b;
-        A[q][k] = sg * a + c * b;
-      }
-      __syncthreads();
-      for (int e = tid; e < half * m; e += nt) {  // columns: A <- A J, V <- V J
-        const int pr = e / m, k = e % m;
-        const int p = P[pr], q = Q[pr];
-        const double c = cs[pr], sg = sn[pr];
-        const double a = A[k][p], b = A[k][q];
-        A[k][p] = c * a - sg * b;
-        A[k][q] = sg * a + c * b;
-        const double va = V[k][p], vb = V[k][q];
-        V[k][p] = c * va - sg * vb;
-        V[k][q] = sg * va + c * vb;
-      }
+      if (k == 0) { d[l] -= p; e[l] = g; e[m] = 0.0; }
       __syncthreads();
     }
-  }
-  // sort the n genuine eigenpairs by value (the dummy index n stays decoupled)
-  for (int i = tid; i < n; i += nt) {
-    const double di = A[i][i];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double dj = A[j][j];
-      rank += (dj < di) || (dj == di && j < i);
-    }
-    w[rank] = di;
-    for (int k = 0; k < n; ++k) S[k * ldS + rank] = V[k][i];
   }
   __syncthreads();
-  __threadfence_block();
-  if (tid == 0) {
-    // status from the sorted output (global memory written by this block)
-    double amax = 0.0;
-    for (int i = 0; i < n; ++i) amax = fmax(amax, fabs(A[i][i]));
-    int imin = 0;
-    for (int i = 1; i < n; ++i) if (A[i][i] < A[imin][imin]) imin = i;
-    // fix the sign: largest component positive
-    int im = 0;
-    for (int k = 1; k < n; ++k) if (fabs(V[k][imin]) > fabs(V[im][imin])) im = k;
-    const double sg = V[im][imin] < 0.0 ? -1.0 : 1.0;
-    for (int k = 0; k < n; ++k) s[k] = sg * V[k][imin];
-    st[0] = A[imin][imin];
-    st[1] = bres ? fabs(*bres * V[n - 1][imin]) : 0.0;
-    st[2] = amax;
+  if (k < n) {
+    const double dk = d[k];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (d[j] < dk) || (d[j] == dk && j < k);
+    w[rank] = dk;
+    for (int row = 0; row < n; ++row) S[row * ldS + rank] = Z[row][k];
   }
 }
 
@@ -969,8 +927,8 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       } else if (J >= mb) {
         // Thick restart keeping the `keep` lowest Ritz pairs in a basis
         // where the projected matrix stays tridiagonal.
-        hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(256), 0, s, k.T, kMaxBasis, J,
-                           k.theta, k.S, kMaxBasis, k.s, k.beta + j, k.st);
+        hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
+                           k.theta, k.S, kMaxBasis);
         TNX_LAUNCH_CHECK();
         int* kp_dev = reinterpret_cast<int*>(k.st + 8);
         hipLaunchKernelGGL(tr_restart_kernel, dim3(1), dim3(64), 0, s, k.theta, k.S, k.beta, j,
