@@ -203,6 +203,72 @@ size_t tnx_dot_workspace_bytes(int64_t n);
 int tnx_dot(int64_t n, const double* x, const double* y, double* out_host,
             void* ws, size_t ws_bytes, tnx_stream_t stream);
 
+/* ---- batched forms: many independent chains of one shape -----------------
+ * The reference runs one FiniteDMRG per chain (one disorder realisation,
+ * tnpy/finite_dmrg.py:24-41 with an MPO from tnpy/model/); sweeping a set of
+ * realisations is a Python loop over those objects. These entry points do
+ * the same site step for `batch` chains of equal length at once: every
+ * operand is a contiguous stack of `batch` per-chain blocks of the size the
+ * single-chain call uses (e.g. L: batch x chil*Dl*chil), and one launch per
+ * kernel covers the whole stack. Results per chain equal those of the
+ * single-chain call on that chain up to rounding. */
+
+/* Stacked tnx_heff1_matvec / tnx_heff1_dense
+ * (tnpy/matrix_product_state.py:411-440 and :372-409, per chain). */
+size_t tnx_heff1_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir, int batch);
+int tnx_heff1_matvec_batched(const double* L, const double* W, const double* R,
+                             const double* x, double* y, int chil, int Dl, int Dr,
+                             int d, int chir, int batch, int precision, void* ws,
+                             size_t ws_bytes, tnx_stream_t stream);
+int tnx_heff1_dense_batched(const double* L, const double* W, const double* R,
+                            double* M, int chil, int Dl, int Dr, int d, int chir,
+                            int batch, tnx_stream_t stream);
+
+/* Stacked tnx_env_left / tnx_env_right
+ * (tnpy/matrix_product_state.py:296-315 and :317-336, per chain). */
+size_t tnx_env_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir, int batch);
+int tnx_env_left_batched(const double* L, const double* A, const double* W,
+                         double* Lout, int chil, int Dl, int Dr, int d, int chir,
+                         int batch, int precision, void* ws, size_t ws_bytes,
+                         tnx_stream_t stream);
+int tnx_env_right_batched(const double* R, const double* A, const double* W,
+                          double* Rout, int chil, int Dl, int Dr, int d, int chir,
+                          int batch, int precision, void* ws, size_t ws_bytes,
+                          tnx_stream_t stream);
+
+/* Stacked tnx_syev_lowest: eval_host receives `batch` eigenvalues
+ * (np.linalg.eigh below exact_solver_dim, tnpy/finite_dmrg.py:108-110). */
+size_t tnx_syev_lowest_batched_workspace_bytes(int n, int batch);
+int tnx_syev_lowest_batched(const double* M, int n, int batch, double* evec,
+                            double* eval_host, void* ws, size_t ws_bytes,
+                            tnx_stream_t stream);
+
+/* Stacked tnx_eigsh_heff1, FP64 only (precision must be TNX_F64, else
+ * TNX_ERR_UNSUPPORTED). Lockstep thick-restart Lanczos: every chain keeps
+ * its own Krylov scalars and leaves the launched set when its own residual
+ * meets the criterion; info receives `batch` records; hvec is required.
+ * (primme.eigsh via tnpy/linalg.py:64-87, called at
+ * tnpy/finite_dmrg.py:111, per chain.) */
+size_t tnx_eigsh_heff1_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir,
+                                               int max_basis, int batch);
+int tnx_eigsh_heff1_batched(const double* L, const double* W, const double* R,
+                            const double* v0, double* vec, double* hvec, int chil,
+                            int Dl, int Dr, int d, int chir, int batch,
+                            const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
+                            void* ws, size_t ws_bytes, tnx_stream_t stream);
+
+/* Stacked tnx_gauge_right / tnx_gauge_left for small bonds: each QR is held
+ * in LDS by one workgroup, so m * (n + 1) <= 16640 (m = chil*d, n = chir
+ * rightward; m = d*chir, n = chil leftward), else TNX_ERR_INVALID.
+ * (tnpy/matrix_product_state.py:200-223 + tnpy/linalg.py:9-23, per chain.) */
+size_t tnx_gauge_batched_workspace_bytes(int chil, int d, int chir, int batch);
+int tnx_gauge_right_batched(double* A, const double* next, double* next_out,
+                            int chil, int d, int chir, int next_cols, int batch,
+                            void* ws, size_t ws_bytes, tnx_stream_t stream);
+int tnx_gauge_left_batched(double* A, const double* prev, double* prev_out,
+                           int chil, int d, int chir, int prev_rows, int batch,
+                           void* ws, size_t ws_bytes, tnx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@ from tnpy_amd.matrix_product_state import (  # noqa: E402
     MatrixProductStateMeasurements,
 )
 from tnpy_amd.finite_dmrg import FiniteDMRG, Metric  # noqa: E402
+from tnpy_amd.batched_dmrg import BatchedFiniteDMRG  # noqa: E402
 
 __all__ = [
     "logger",
@@ -40,4 +41,5 @@ __all__ = [
     "Direction",
     "FiniteDMRG",
     "Metric",
+    "BatchedFiniteDMRG",
 ]

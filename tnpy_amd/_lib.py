@@ -71,6 +71,22 @@ SIGNATURES = {
         _i,
         [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, C.POINTER(EigshOpts), C.POINTER(EigshInfo), _p, _sz, _p],
     ),
+    "tnx_heff1_batched_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "tnx_heff1_matvec_batched": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_heff1_dense_batched": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
+    "tnx_env_batched_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "tnx_env_left_batched": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_env_right_batched": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_syev_lowest_batched_workspace_bytes": (_sz, [_i, _i]),
+    "tnx_syev_lowest_batched": (_i, [_p, _i, _i, _p, C.POINTER(_d), _p, _sz, _p]),
+    "tnx_eigsh_heff1_batched_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
+    "tnx_eigsh_heff1_batched": (
+        _i,
+        [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, C.POINTER(EigshOpts), C.POINTER(EigshInfo), _p, _sz, _p],
+    ),
+    "tnx_gauge_batched_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "tnx_gauge_right_batched": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_gauge_left_batched": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _sz, _p]),
     "tnx_heff2_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i]),
     "tnx_heff2_matvec": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
     "tnx_eigsh_heff2_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i]),

@@ -1,0 +1,375 @@
+"""One-site finite DMRG of many independent chains at once.
+
+The reference sweeps one chain per ``FiniteDMRG`` object
+(``tnpy/finite_dmrg.py:24-41``, ``sweep`` at ``:143-171``, ``run`` at
+``:214-257``); a study of many disorder realisations is a Python loop over
+such objects. ``BatchedFiniteDMRG`` runs the same algorithm for a list of
+chains: chains of equal length share every tensor shape at every site, so
+each length group keeps its tensors as stacks (chain index first) and runs
+each site step for the whole group with one launch per kernel:
+
+- the lowest eigenpair of every chain's H_eff (batched dense eigensolver
+  below ``exact_solver_dim``, lockstep batched Lanczos otherwise, where each
+  chain converges on its own residual),
+- psi <- psi + 1e-5 H psi,
+- the QR / LQ gauge move (one workgroup per chain, LDS resident),
+- the environment update.
+
+Per chain the result equals a ``FiniteDMRG`` run from the same initial
+state up to rounding. FP64 throughout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from itertools import cycle
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from tnpy_amd import _lib, logger
+from tnpy_amd.matrix_product_state import Direction, MatrixProductState
+from tnpy_amd.operators import as_mpo
+
+
+class _Group:
+    """Stacked state of the chains of one length."""
+
+    def __init__(self, owner: "BatchedFiniteDMRG", chains: List[int], mpos, starts):
+        self.owner = owner
+        self.chains = chains
+        self.B = len(chains)
+        dev = owner.device
+        self.n = mpos[0].n_sites
+        self.W = [
+            torch.stack([torch.as_tensor(m.padded[s], device=dev) for m in mpos]).contiguous()
+            for s in range(self.n)
+        ]
+        self.A = [torch.stack([st[s] for st in starts]).contiguous() for s in range(self.n)]
+        one = torch.ones(self.B, 1, 1, 1, dtype=torch.float64, device=dev)
+        self.left: Dict[int, torch.Tensor] = {0: one}
+        self.right: Dict[int, torch.Tensor] = {self.n - 1: one.clone()}
+        for site in range(self.n - 2, -1, -1):
+            self.update_right(site)
+
+    # -- building blocks -------------------------------------------------------
+    def dims(self, site: int):
+        _, l, d, r = self.A[site].shape
+        W = self.W[site]
+        return l, int(W.shape[1]), int(W.shape[2]), d, r
+
+    def update_left(self, site: int) -> None:
+        L, A, W = self.left[site - 1], self.A[site - 1], self.W[site - 1]
+        l, Dl, Dr, d, r = self.dims(site - 1)
+        out = torch.empty(self.B, r, Dr, r, dtype=torch.float64, device=A.device)
+        lib = _lib.lib()
+        wp, wb = self.owner._ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        _lib.check(
+            lib.tnx_env_left_batched(
+                _lib.ptr(L, "L"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Lout"),
+                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.owner._stream(),
+            ),
+            "tnx_env_left_batched",
+        )
+        self.left[site] = out
+
+    def update_right(self, site: int) -> None:
+        R, A, W = self.right[site + 1], self.A[site + 1], self.W[site + 1]
+        l, Dl, Dr, d, r = self.dims(site + 1)
+        out = torch.empty(self.B, l, Dl, l, dtype=torch.float64, device=A.device)
+        lib = _lib.lib()
+        wp, wb = self.owner._ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        _lib.check(
+            lib.tnx_env_right_batched(
+                _lib.ptr(R, "R"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Rout"),
+                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.owner._stream(),
+            ),
+            "tnx_env_right_batched",
+        )
+        self.right[site] = out
+
+    def ensure_left(self, site: int) -> None:
+        first = max(k for k in self.left if k <= site)
+        for s in range(first + 1, site + 1):
+            self.update_left(s)
+
+    def matvec(self, site: int, x: torch.Tensor, y: torch.Tensor) -> None:
+        l, Dl, Dr, d, r = self.dims(site)
+        lib = _lib.lib()
+        wp, wb = self.owner._ws.get(lib.tnx_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        _lib.check(
+            lib.tnx_heff1_matvec_batched(
+                _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
+                _lib.ptr(x, "x"), _lib.ptr(y, "y"), l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb,
+                self.owner._stream(),
+            ),
+            "tnx_heff1_matvec_batched",
+        )
+
+    def solve(self, site: int, tol: float):
+        """Lowest eigenpair of every chain's H_eff at ``site``: (energies,
+        vec, hvec), vec/hvec stacked (B, N)."""
+        o = self.owner
+        lib = _lib.lib()
+        l, Dl, Dr, d, r = self.dims(site)
+        N = l * d * r
+        dev = o.device
+        vec = torch.empty(self.B, N, dtype=torch.float64, device=dev)
+        hvec = torch.empty(self.B, N, dtype=torch.float64, device=dev)
+        evals = (C.c_double * self.B)()
+        if N < o._exact_solver_dim:
+            M = torch.empty(self.B, N, N, dtype=torch.float64, device=dev)
+            _lib.check(
+                lib.tnx_heff1_dense_batched(
+                    _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
+                    _lib.ptr(M, "M"), l, Dl, Dr, d, r, self.B, o._stream(),
+                ),
+                "tnx_heff1_dense_batched",
+            )
+            wp, wb = o._ws.get(lib.tnx_syev_lowest_batched_workspace_bytes(N, self.B))
+            _lib.check(
+                lib.tnx_syev_lowest_batched(_lib.ptr(M, "M"), N, self.B, _lib.ptr(vec, "evec"), evals, wp, wb,
+                                            o._stream()),
+                "tnx_syev_lowest_batched",
+            )
+            self.matvec(site, vec, hvec)
+            o.stats["dense_sites"] += 1
+            return np.array(evals[:]), vec, hvec
+        opts = _lib.EigshOpts()
+        opts.tol = float(tol)
+        opts.max_basis = int(o._max_basis)
+        opts.max_matvecs = int(o._max_matvecs)
+        opts.precision = _lib.TNX_F64
+        opts.keep = int(o._keep)
+        infos = (_lib.EigshInfo * self.B)()
+        wp, wb = o._ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, self.B))
+        v0 = self.A[site].reshape(self.B, N)
+        st = lib.tnx_eigsh_heff1_batched(
+            _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
+            _lib.ptr(v0, "v0"), _lib.ptr(vec, "vec"), _lib.ptr(hvec, "hvec"), l, Dl, Dr, d, r, self.B,
+            C.byref(opts), infos, wp, wb, o._stream(),
+        )
+        if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
+            _lib.check(st, "tnx_eigsh_heff1_batched")
+        if st == _lib.TNX_ERR_NOCONV:
+            bad = [self.chains[i] for i in range(self.B) if not infos[i].converged]
+            logger.warning(f"Lanczos at site {site} hit the matvec limit for chains {bad}.")
+        o.stats["krylov_sites"] += 1
+        o.stats["matvecs"] += sum(infos[i].matvecs for i in range(self.B))
+        o.stats["lockstep_matvecs"] += max(infos[i].matvecs for i in range(self.B))
+        o.stats["restarts"] += max(infos[i].restarts for i in range(self.B))
+        return np.array([infos[i].eval for i in range(self.B)]), vec, hvec
+
+    def gauge(self, site: int, direction: Direction) -> None:
+        lib = _lib.lib()
+        A = self.A[site]
+        _, l, p, r = A.shape
+        wp, wb = self.owner._ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, self.B))
+        if direction == Direction.RIGHTWARD:
+            nxt = self.A[site + 1]
+            out = torch.empty_like(nxt)
+            _lib.check(
+                lib.tnx_gauge_right_batched(
+                    _lib.ptr(A, "site"), _lib.ptr(nxt, "next"), _lib.ptr(out, "next_out"),
+                    l, p, r, nxt.shape[2] * nxt.shape[3], self.B, wp, wb, self.owner._stream(),
+                ),
+                "tnx_gauge_right_batched",
+            )
+            self.A[site + 1] = out
+        else:
+            prv = self.A[site - 1]
+            out = torch.empty_like(prv)
+            _lib.check(
+                lib.tnx_gauge_left_batched(
+                    _lib.ptr(A, "site"), _lib.ptr(prv, "prev"), _lib.ptr(out, "prev_out"),
+                    l, p, r, prv.shape[1] * prv.shape[2], self.B, wp, wb, self.owner._stream(),
+                ),
+                "tnx_gauge_left_batched",
+            )
+            self.A[site - 1] = out
+
+    # -- site step and sweep -------------------------------------------------------
+    def site_step(self, site: int, direction: Direction, tol: float) -> np.ndarray:
+        if site > 0 and site not in self.left:
+            self.ensure_left(site)
+        energies, vec, hvec = self.solve(site, tol)
+        _lib.check(
+            _lib.lib().tnx_axpy(vec.numel(), 1e-5, _lib.ptr(hvec, "hvec"), _lib.ptr(vec, "vec"),
+                                self.owner._stream()),
+            "tnx_axpy",
+        )
+        self.A[site] = vec.reshape(self.A[site].shape)
+        self.gauge(site, direction)
+        if direction == Direction.RIGHTWARD:
+            self.update_left(site + 1)
+        else:
+            self.update_right(site - 1)
+        return energies
+
+    def sweep(self, direction: Direction, tol: float) -> np.ndarray:
+        if direction == Direction.LEFTWARD:
+            self.ensure_left(self.n - 1)
+        sites = range(self.n - 1) if direction == Direction.RIGHTWARD else range(self.n - 1, 0, -1)
+        energies = None
+        for site in sites:
+            energies = self.site_step(site, direction, tol)
+        return energies
+
+
+class BatchedFiniteDMRG:
+    """One-site finite DMRG of a list of independent chains.
+
+    Args:
+        mpos: one MPO per chain (``MatrixProductOperator``, W arrays or a
+            quimb MPO). Lengths may differ; the physical dimension may not.
+        bond_dim: bond dimension of every chain (``chi`` is an alias).
+        mps: optional initial states, one per chain (``MatrixProductState``
+            or arrays, 'lpr' site tensors).
+        seeds: seeds of the random initial states (default 0, 1, 2, ...),
+            each drawn as ``MatrixProductState.random(n, chi, d, seed)``.
+        exact_solver_dim, max_basis, keep: as for ``FiniteDMRG``.
+        max_matvecs: Lanczos matvec limit per site.
+    """
+
+    def __init__(
+        self,
+        mpos: Sequence,
+        bond_dim: Optional[int] = None,
+        mps: Optional[Sequence] = None,
+        seeds: Optional[Sequence[int]] = None,
+        exact_solver_dim: int = 200,
+        *,
+        chi: Optional[int] = None,
+        max_basis: int = 64,
+        keep: int = 20,
+        max_matvecs: int = 5000,
+        device: Optional[torch.device] = None,
+    ):
+        if chi is not None and bond_dim is not None and chi != bond_dim:
+            raise ValueError("chi and bond_dim disagree")
+        bond = chi if chi is not None else bond_dim
+        if bond is None:
+            raise TypeError("BatchedFiniteDMRG needs a bond dimension (bond_dim or chi)")
+        if len(mpos) == 0:
+            raise ValueError("no chains")
+        self.device = device if device is not None else _lib.require_gpu()
+        _lib.load_library()
+        self._mpos = [as_mpo(m) for m in mpos]
+        d = self._mpos[0].phys_dim
+        if any(m.phys_dim != d for m in self._mpos):
+            raise ValueError("all chains need the same physical dimension")
+        self._bond_dim = int(bond)
+        self._phys_dim = d
+        self._exact_solver_dim = exact_solver_dim
+        self._max_basis = max_basis
+        self._keep = keep
+        self._max_matvecs = max_matvecs
+        self._ws = _lib.Workspace(self.device)
+        self.stats = {"matvecs": 0, "lockstep_matvecs": 0, "restarts": 0, "dense_sites": 0, "krylov_sites": 0}
+        nchain = len(self._mpos)
+        if mps is not None and len(mps) != nchain:
+            raise ValueError("need one initial state per chain")
+        if seeds is None:
+            seeds = list(range(nchain))
+        starts = []
+        for i, m in enumerate(self._mpos):
+            if mps is None:
+                st = MatrixProductState.random(m.n_sites, self._bond_dim, d, seed=int(seeds[i]), device=self.device)
+            elif isinstance(mps[i], MatrixProductState):
+                st = mps[i]
+            else:
+                st = MatrixProductState(mps[i], device=self.device)
+            if st.n_sites != m.n_sites:
+                raise ValueError(f"chain {i}: MPO and MPS lengths differ")
+            starts.append(st.tensors)
+        by_len: Dict[int, List[int]] = {}
+        for i, m in enumerate(self._mpos):
+            by_len.setdefault(m.n_sites, []).append(i)
+        self._groups = []
+        for n in sorted(by_len):
+            idx = by_len[n]
+            shapes = {tuple(tuple(t.shape) for t in starts[i]) for i in idx}
+            if len(shapes) != 1:
+                raise ValueError(f"chains of length {n} have different bond dimensions")
+            self._groups.append(_Group(self, idx, [self._mpos[i] for i in idx], [starts[i] for i in idx]))
+        self._energies: List[List[float]] = [[] for _ in range(nchain)]
+        self.sweep_times: List[float] = []
+
+    def _stream(self) -> int:
+        return _lib.stream_ptr(self.device)
+
+    @property
+    def n_chains(self) -> int:
+        return len(self._mpos)
+
+    @property
+    def bond_dim(self) -> int:
+        return self._bond_dim
+
+    @property
+    def groups(self) -> Dict[int, List[int]]:
+        """Chain indices per chain length."""
+        return {g.n: list(g.chains) for g in self._groups}
+
+    def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8) -> np.ndarray:
+        """One directional sweep of every chain; returns each chain's
+        energy at the last site of the sweep."""
+        out = np.empty(self.n_chains)
+        for g in self._groups:
+            e = g.sweep(direction, tol)
+            out[g.chains] = e
+        return out
+
+    def run(self, tol: float = 1e-8, max_sweep: int = 100) -> List[List[float]]:
+        """Sweep all chains (alternating directions) until every chain's
+        energy changes by less than ``tol`` between sweeps or ``max_sweep``
+        is reached (the stopping rule of ``tnpy/finite_dmrg.py:173-212`` per
+        chain). Returns the per-sweep energies of every chain."""
+        done = np.zeros(self.n_chains, dtype=bool)
+        prev = np.full(self.n_chains, np.nan)
+        for n_sweep, direction in zip(range(1, max_sweep + 1), cycle([Direction.RIGHTWARD, Direction.LEFTWARD])):
+            t0 = time.perf_counter()
+            e = self.sweep(direction, tol)
+            torch.cuda.synchronize(self.device)
+            self.sweep_times.append(time.perf_counter() - t0)
+            for i in range(self.n_chains):
+                if not done[i]:
+                    self._energies[i].append(float(e[i]))
+            grad = np.abs(e - prev)
+            done |= grad < tol
+            prev = e
+            logger.info(
+                f"Batched sweep {n_sweep}/{max_sweep}: {int(done.sum())}/{self.n_chains} chains converged, "
+                f"{self.sweep_times[-1]:.3f} s"
+            )
+            if done.all():
+                break
+        return self.energies
+
+    update = run
+
+    @property
+    def energies(self) -> List[List[float]]:
+        return [list(e) for e in self._energies]
+
+    def mps(self, chain: int) -> MatrixProductState:
+        """The current state of one chain as a ``MatrixProductState``."""
+        for g in self._groups:
+            if chain in g.chains:
+                k = g.chains.index(chain)
+                return MatrixProductState([a[k].clone() for a in g.A], device=self.device)
+        raise IndexError(chain)
+
+
+def random_coupling_chains(n_chains: int, lengths: Sequence[int], seed: int):
+    """Chain lengths drawn uniformly from ``lengths`` and per-chain coupling
+    seeds, all from ``numpy.random.default_rng(seed)``; returns
+    (lengths, coupling seeds) for ``RandomCouplingHeisenberg``."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(lengths), size=n_chains)
+    seeds = rng.integers(0, 2**31 - 1, size=n_chains)
+    return [int(n) for n in ns], [int(s) for s in seeds]
+
+
+__all__ = ["BatchedFiniteDMRG", "random_coupling_chains"]

@@ -33,12 +33,22 @@ struct WApply {
   int64_t out_so, out_sp, out_sq;  // strides of o, p, q in `out`
   int NX, NY, NP, NQ;
   int Dl, Dr, d, mode;
+  // batch of independent problems (grid.y): strides of in, out and W
+  int batch = 1;
+  int64_t in_sb = 0, out_sb = 0, W_sb = 0;
+  const int* bidx = nullptr;  // launched -> logical problem, or null
 };
 
 template <int MAXIN>
 __global__ __launch_bounds__(256) void wapply_kernel(WApply a) {
   __shared__ double C[64 * 64];
   const int nin = a.NX * a.NY, nout = a.NP * a.NQ;
+  {
+    const int b = a.bidx ? a.bidx[blockIdx.y] : (int)blockIdx.y;
+    a.in = (const char*)a.in + (size_t)b * a.in_sb * (a.in_f32 ? 4 : 8);
+    a.out = (char*)a.out + (size_t)b * a.out_sb * (a.out_f32 ? 4 : 8);
+    a.W += (size_t)b * a.W_sb;
+  }
   for (int t = threadIdx.x; t < nout * nin; t += blockDim.x) {
     const int oi = t / nin, ii = t % nin;
     const int p = oi / a.NQ, q = oi % a.NQ;
@@ -82,11 +92,12 @@ static int launch_wapply(const WApply& a, hipStream_t s) {
                 "W-apply: D*d = %d / %d exceeds 64", a.NP * a.NQ, nin);
   const int64_t total = a.n_outer * a.n_inner;
   if (total == 0) return TNX_OK;
-  int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  TNX_CHECK_ARG(a.batch >= 1 && a.batch <= 65535, "W-apply: bad batch %d", a.batch);
+  int blocks = (int)std::min<int64_t>((total + 255) / 256, std::max<int64_t>(8192 / a.batch, 64));
   if (nin <= 16)
-    hipLaunchKernelGGL(wapply_kernel<16>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(wapply_kernel<16>, dim3(blocks, a.batch), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL(wapply_kernel<64>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(wapply_kernel<64>, dim3(blocks, a.batch), dim3(256), 0, s, a);
   TNX_LAUNCH_CHECK();
   return TNX_OK;
 }
@@ -103,31 +114,45 @@ static size_t inter_bytes(int prec) { return prec == TNX_F64 ? 8 : 4; }
 // long; the split rule in gemm.hip decides how many are used).
 static size_t splitk_bytes(int64_t out_elems) { return (size_t)out_elems * 2 * 8; }
 
-size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
+size_t heff1_ws_bytes_b(int chil, int Dl, int Dr, int d, int chir, int batch) {
   const size_t t1 = (size_t)chil * d * Dr * chir;
   const size_t t2 = (size_t)chil * Dl * d * chir;
-  return (t1 + t2) * 8 + 512 + splitk_bytes((int64_t)chil * d * chir);
+  return ((t1 + t2) * 8 + splitk_bytes((int64_t)chil * d * chir)) * (size_t)batch + 512;
 }
 
-int heff1_matvec(const double* L, const double* W, const double* R,
-                 const double* x, double* y, int chil, int Dl, int Dr, int d,
-                 int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s) {
+size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
+  return heff1_ws_bytes_b(chil, Dl, Dr, d, chir, 1);
+}
+
+// y_p = H_eff(L_p, W_p, R_p) x_p for the problems p of a stack of `batch`:
+// all of them (bidx null) or the `count` listed in bidx. L, W, R are
+// contiguous stacks; x_p = x + p*sx, y_p = y + p*sy (sx = sy = chil*d*chir
+// for contiguous stacks). One problem: batch = count = 1.
+int heff1_matvec_b(const double* L, const double* W, const double* R, const double* x, double* y,
+                   int chil, int Dl, int Dr, int d, int chir, int prec, int batch, int count,
+                   const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
+                   hipStream_t s) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "heff1_matvec: bad dims");
+  TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "heff1_matvec: bad batch %d", batch);
+  TNX_CHECK_ARG(count >= 1 && count <= batch && (bidx || count == batch),
+                "heff1_matvec: bad count %d", count);
   TNX_CHECK_ARG(L && W && R && x && y, "heff1_matvec: null pointer");
   TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "heff1_matvec: bad precision");
-  TNX_CHECK_ARG(ws && ws_bytes >= heff1_ws_bytes(chil, Dl, Dr, d, chir),
+  TNX_CHECK_ARG(ws && ws_bytes >= heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch),
                 "heff1_matvec: workspace too small");
   const size_t eb = inter_bytes(prec);
   const DT it = prec == TNX_F64 ? DT::F64 : DT::F32;
+  const int64_t nx = (int64_t)chil * d * chir;
+  const int64_t nt1 = (int64_t)chil * d * Dr * chir, nt2 = (int64_t)chil * Dl * d * chir;
   char* T1 = (char*)ws;
-  char* T2 = T1 + (((size_t)chil * d * Dr * chir * eb + 255) & ~(size_t)255);
-  char* P = T2 + (((size_t)chil * Dl * d * chir * eb + 255) & ~(size_t)255);
+  char* T2 = T1 + (((size_t)nt1 * batch * eb + 255) & ~(size_t)255);
+  char* P = T2 + (((size_t)nt2 * batch * eb + 255) & ~(size_t)255);
   // T1[(a,k),(v,s)] = X[(a,k), r] R[r, (v,s)]
   GemmDesc g1{};
-  g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = 1;
-  g1.A = x; g1.ta = DT::F64; g1.lda = chir; g1.sA = 0; g1.a_kcontig = true;
-  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = 0; g1.b_ncontig = true;
-  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = 0;
+  g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = count; g1.bidx = bidx;
+  g1.A = x; g1.ta = DT::F64; g1.lda = chir; g1.sA = sx; g1.a_kcontig = true;
+  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = (int64_t)chir * Dr * chir; g1.b_ncontig = true;
+  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = nt1;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
   TNX_TRY(gemm(g1, s));
   WApply wa{};
@@ -137,17 +162,27 @@ int heff1_matvec(const double* L, const double* W, const double* R,
   wa.out_so = (int64_t)Dl * d * chir; wa.out_sp = (int64_t)d * chir; wa.out_sq = chir;
   wa.NX = d; wa.NY = Dr; wa.NP = Dl; wa.NQ = d;
   wa.Dl = Dl; wa.Dr = Dr; wa.d = d; wa.mode = 0;
+  wa.batch = count; wa.bidx = bidx;
+  wa.in_sb = nt1; wa.out_sb = nt2; wa.W_sb = (int64_t)Dl * Dr * d * d;
   TNX_TRY(launch_wapply(wa, s));
   // y[b, (c,s)] = sum_{(a,w)} L[(a,w), b] T2[(a,w), (c,s)]
   GemmDesc g3{};
-  g3.M = chil; g3.N = d * chir; g3.K = chil * Dl; g3.batch = 1;
-  g3.A = L; g3.ta = DT::F64; g3.lda = chil; g3.sA = 0; g3.a_kcontig = false;
-  g3.B = T2; g3.tb = it; g3.ldb = (int64_t)d * chir; g3.sB = 0; g3.b_ncontig = true;
-  g3.C = y; g3.tc = DT::F64; g3.ldc = (int64_t)d * chir; g3.sC = 0;
+  g3.M = chil; g3.N = d * chir; g3.K = chil * Dl; g3.batch = count; g3.bidx = bidx;
+  g3.A = L; g3.ta = DT::F64; g3.lda = chil; g3.sA = (int64_t)chil * Dl * chil; g3.a_kcontig = false;
+  g3.B = T2; g3.tb = it; g3.ldb = (int64_t)d * chir; g3.sB = nt2; g3.b_ncontig = true;
+  g3.C = y; g3.tc = DT::F64; g3.ldc = (int64_t)d * chir; g3.sC = sy;
   g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
   g3.splitk_ws = reinterpret_cast<double*>(P);
-  g3.splitk_ws_bytes = splitk_bytes((int64_t)chil * d * chir);
+  g3.splitk_ws_bytes = splitk_bytes(nx) * batch;
   return gemm(g3, s);
+}
+
+int heff1_matvec(const double* L, const double* W, const double* R,
+                 const double* x, double* y, int chil, int Dl, int Dr, int d,
+                 int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s) {
+  const int64_t nx = (int64_t)chil * d * chir;
+  return heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, prec, 1, 1, nullptr, nx, nx, ws,
+                        ws_bytes, s);
 }
 
 // Two-site effective Hamiltonian (no reference function: the two-site
@@ -212,65 +247,87 @@ int heff2_matvec(const double* L, const double* W1, const double* W2, const doub
   return gemm(g3, s);
 }
 
-size_t env_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
-  return (size_t)chil * d * chir * (Dl + Dr) * 8 + 256;
+size_t env_ws_bytes_b(int chil, int Dl, int Dr, int d, int chir, int batch) {
+  return (size_t)chil * d * chir * (Dl + Dr) * 8 * (size_t)batch + 512;
 }
 
-int env_left(const double* L, const double* A, const double* W, double* Lout,
-             int chil, int Dl, int Dr, int d, int chir, int prec, void* ws,
-             size_t ws_bytes, hipStream_t s) {
+size_t env_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
+  return env_ws_bytes_b(chil, Dl, Dr, d, chir, 1);
+}
+
+// Left environment update for a contiguous stack of `batch` problems
+// (L, A, W, Lout each a stack of per-problem blocks).
+int env_left_b(const double* L, const double* A, const double* W, double* Lout,
+               int chil, int Dl, int Dr, int d, int chir, int prec, int batch, void* ws,
+               size_t ws_bytes, hipStream_t s) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "env_left: bad dims");
+  TNX_CHECK_ARG(batch >= 1 && (int64_t)batch * Dr <= 65535, "env_left: bad batch %d", batch);
   TNX_CHECK_ARG(L && A && W && Lout, "env_left: null pointer");
   TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "env_left: bad precision");
-  TNX_CHECK_ARG(ws && ws_bytes >= env_ws_bytes(chil, Dl, Dr, d, chir),
+  TNX_CHECK_ARG(ws && ws_bytes >= env_ws_bytes_b(chil, Dl, Dr, d, chir, batch),
                 "env_left: workspace too small");
   const size_t eb = inter_bytes(prec);
   const DT it = prec == TNX_F64 ? DT::F64 : DT::F32;
+  const int64_t nA = (int64_t)chil * d * chir;
+  const int64_t nT1 = (int64_t)Dl * nA, nT2 = (int64_t)Dr * nA;
   char* T1 = (char*)ws;
-  char* T2 = T1 + (((size_t)Dl * chil * d * chir * eb + 255) & ~(size_t)255);
+  char* T2 = T1 + (((size_t)nT1 * batch * eb + 255) & ~(size_t)255);
   // T1[(w,b),(k,r)] = sum_a L[a,(w,b)] A[a,(k,r)]
   GemmDesc g1{};
-  g1.M = Dl * chil; g1.N = d * chir; g1.K = chil; g1.batch = 1;
-  g1.A = L; g1.ta = DT::F64; g1.lda = (int64_t)Dl * chil; g1.sA = 0; g1.a_kcontig = false;
-  g1.B = A; g1.tb = DT::F64; g1.ldb = (int64_t)d * chir; g1.sB = 0; g1.b_ncontig = true;
-  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)d * chir; g1.sC = 0;
+  g1.M = Dl * chil; g1.N = d * chir; g1.K = chil; g1.batch = batch;
+  g1.A = L; g1.ta = DT::F64; g1.lda = (int64_t)Dl * chil; g1.sA = (int64_t)chil * Dl * chil; g1.a_kcontig = false;
+  g1.B = A; g1.tb = DT::F64; g1.ldb = (int64_t)d * chir; g1.sB = nA; g1.b_ncontig = true;
+  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)d * chir; g1.sC = nT1;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
   TNX_TRY(gemm(g1, s));
   WApply wa{};
   wa.in = T1; wa.out = T2; wa.W = W; wa.in_f32 = wa.out_f32 = (prec != TNX_F64);
   wa.n_outer = chil; wa.n_inner = chir;
-  wa.in_so = (int64_t)d * chir; wa.in_sx = (int64_t)chil * d * chir; wa.in_sy = chir;
-  wa.out_so = (int64_t)d * chir; wa.out_sp = (int64_t)chil * d * chir; wa.out_sq = chir;
+  wa.in_so = (int64_t)d * chir; wa.in_sx = nA; wa.in_sy = chir;
+  wa.out_so = (int64_t)d * chir; wa.out_sp = nA; wa.out_sq = chir;
   wa.NX = Dl; wa.NY = d; wa.NP = Dr; wa.NQ = d;
   wa.Dl = Dl; wa.Dr = Dr; wa.d = d; wa.mode = 1;
+  wa.batch = batch; wa.in_sb = nT1; wa.out_sb = nT2; wa.W_sb = (int64_t)Dl * Dr * d * d;
   TNX_TRY(launch_wapply(wa, s));
-  // L'[r, v, s] = sum_{(b,c)} T2[v][(b,c)][r] A[(b,c)][s], batched over v
+  // L'[r, v, s] = sum_{(b,c)} T2[v][(b,c)][r] A[(b,c)][s], batched over
+  // (problem, v)
   GemmDesc g3{};
-  g3.M = chir; g3.N = chir; g3.K = chil * d; g3.batch = Dr;
-  g3.A = T2; g3.ta = it; g3.lda = chir; g3.sA = (int64_t)chil * d * chir; g3.a_kcontig = false;
-  g3.B = A; g3.tb = DT::F64; g3.ldb = chir; g3.sB = 0; g3.b_ncontig = true;
-  g3.C = Lout; g3.tc = DT::F64; g3.ldc = (int64_t)Dr * chir; g3.sC = chir;
+  g3.M = chir; g3.N = chir; g3.K = chil * d; g3.batch = batch * Dr; g3.nb2 = Dr;
+  g3.A = T2; g3.ta = it; g3.lda = chir; g3.sA = nT2; g3.sA2 = nA; g3.a_kcontig = false;
+  g3.B = A; g3.tb = DT::F64; g3.ldb = chir; g3.sB = nA; g3.sB2 = 0; g3.b_ncontig = true;
+  g3.C = Lout; g3.tc = DT::F64; g3.ldc = (int64_t)Dr * chir; g3.sC = (int64_t)chir * Dr * chir;
+  g3.sC2 = chir;
   g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
   return gemm(g3, s);
 }
 
-int env_right(const double* R, const double* A, const double* W, double* Rout,
-              int chil, int Dl, int Dr, int d, int chir, int prec, void* ws,
-              size_t ws_bytes, hipStream_t s) {
+int env_left(const double* L, const double* A, const double* W, double* Lout,
+             int chil, int Dl, int Dr, int d, int chir, int prec, void* ws,
+             size_t ws_bytes, hipStream_t s) {
+  return env_left_b(L, A, W, Lout, chil, Dl, Dr, d, chir, prec, 1, ws, ws_bytes, s);
+}
+
+// Right environment update for a contiguous stack of `batch` problems.
+int env_right_b(const double* R, const double* A, const double* W, double* Rout,
+                int chil, int Dl, int Dr, int d, int chir, int prec, int batch, void* ws,
+                size_t ws_bytes, hipStream_t s) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "env_right: bad dims");
+  TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "env_right: bad batch %d", batch);
   TNX_CHECK_ARG(R && A && W && Rout, "env_right: null pointer");
   TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "env_right: bad precision");
-  TNX_CHECK_ARG(ws && ws_bytes >= env_ws_bytes(chil, Dl, Dr, d, chir),
+  TNX_CHECK_ARG(ws && ws_bytes >= env_ws_bytes_b(chil, Dl, Dr, d, chir, batch),
                 "env_right: workspace too small");
   const size_t eb = inter_bytes(prec);
   const DT it = prec == TNX_F64 ? DT::F64 : DT::F32;
+  const int64_t nA = (int64_t)chil * d * chir;
+  const int64_t nT1 = (int64_t)Dr * nA, nT2 = (int64_t)Dl * nA;
   char* T1 = (char*)ws;
-  char* T2 = T1 + (((size_t)chil * d * Dr * chir * eb + 255) & ~(size_t)255);
+  char* T2 = T1 + (((size_t)nT1 * batch * eb + 255) & ~(size_t)255);
   GemmDesc g1{};
-  g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = 1;
-  g1.A = A; g1.ta = DT::F64; g1.lda = chir; g1.sA = 0; g1.a_kcontig = true;
-  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = 0; g1.b_ncontig = true;
-  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = 0;
+  g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = batch;
+  g1.A = A; g1.ta = DT::F64; g1.lda = chir; g1.sA = nA; g1.a_kcontig = true;
+  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = (int64_t)chir * Dr * chir; g1.b_ncontig = true;
+  g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = nT1;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
   TNX_TRY(gemm(g1, s));
   WApply wa{};
@@ -280,21 +337,36 @@ int env_right(const double* R, const double* A, const double* W, double* Rout,
   wa.out_so = (int64_t)Dl * d * chir; wa.out_sp = (int64_t)d * chir; wa.out_sq = chir;
   wa.NX = d; wa.NY = Dr; wa.NP = Dl; wa.NQ = d;
   wa.Dl = Dl; wa.Dr = Dr; wa.d = d; wa.mode = 0;
+  wa.batch = batch; wa.in_sb = nT1; wa.out_sb = nT2; wa.W_sb = (int64_t)Dl * Dr * d * d;
   TNX_TRY(launch_wapply(wa, s));
   // R'[(a,w), b] = sum_{(c,s)} T2[(a,w),(c,s)] A[b,(c,s)]
   GemmDesc g3{};
-  g3.M = chil * Dl; g3.N = chil; g3.K = d * chir; g3.batch = 1;
-  g3.A = T2; g3.ta = it; g3.lda = (int64_t)d * chir; g3.sA = 0; g3.a_kcontig = true;
-  g3.B = A; g3.tb = DT::F64; g3.ldb = (int64_t)d * chir; g3.sB = 0; g3.b_ncontig = false;
-  g3.C = Rout; g3.tc = DT::F64; g3.ldc = chil; g3.sC = 0;
+  g3.M = chil * Dl; g3.N = chil; g3.K = d * chir; g3.batch = batch;
+  g3.A = T2; g3.ta = it; g3.lda = (int64_t)d * chir; g3.sA = nT2; g3.a_kcontig = true;
+  g3.B = A; g3.tb = DT::F64; g3.ldb = (int64_t)d * chir; g3.sB = nA; g3.b_ncontig = false;
+  g3.C = Rout; g3.tc = DT::F64; g3.ldc = chil; g3.sC = (int64_t)chil * Dl * chil;
   g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
   return gemm(g3, s);
 }
 
+int env_right(const double* R, const double* A, const double* W, double* Rout,
+              int chil, int Dl, int Dr, int d, int chir, int prec, void* ws,
+              size_t ws_bytes, hipStream_t s) {
+  return env_right_b(R, A, W, Rout, chil, Dl, Dr, d, chir, prec, 1, ws, ws_bytes, s);
+}
+
 // M[(a,k,r),(b,c,s)] = sum_{w,v} L[a,w,b] W[w,v,k,c] R[r,v,s]
+// Stacks over blockIdx.y: problem p reads L, W, R, M + p * (their sizes).
 __global__ void heff_dense_kernel(const double* L, const double* W, const double* R,
                                   double* M, int chil, int Dl, int Dr, int d, int chir) {
   const int64_t N = (int64_t)chil * d * chir;
+  {
+    const int64_t p = blockIdx.y;
+    L += p * chil * Dl * chil;
+    W += p * Dl * Dr * d * d;
+    R += p * chir * Dr * chir;
+    M += p * N * N;
+  }
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < N * N;
        g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = g / N, col = g % N;
@@ -311,16 +383,23 @@ __global__ void heff_dense_kernel(const double* L, const double* W, const double
   }
 }
 
-int heff1_dense(const double* L, const double* W, const double* R, double* M,
-                int chil, int Dl, int Dr, int d, int chir, hipStream_t s) {
+int heff1_dense_b(const double* L, const double* W, const double* R, double* M,
+                  int chil, int Dl, int Dr, int d, int chir, int batch, hipStream_t s) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "heff1_dense: bad dims");
+  TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "heff1_dense: bad batch %d", batch);
   TNX_CHECK_ARG(L && W && R && M, "heff1_dense: null pointer");
   const int64_t N = (int64_t)chil * d * chir;
   TNX_CHECK_ARG(N <= 8192, "heff1_dense: dimension %lld too large for a dense H_eff", (long long)N);
-  const int blocks = (int)std::min<int64_t>((N * N + 255) / 256, 4096);
-  hipLaunchKernelGGL(heff_dense_kernel, dim3(blocks), dim3(256), 0, s, L, W, R, M, chil, Dl, Dr, d, chir);
+  const int blocks = (int)std::min<int64_t>((N * N + 255) / 256, std::max(4096 / batch, 16));
+  hipLaunchKernelGGL(heff_dense_kernel, dim3(blocks, batch), dim3(256), 0, s, L, W, R, M, chil,
+                     Dl, Dr, d, chir);
   TNX_LAUNCH_CHECK();
   return TNX_OK;
+}
+
+int heff1_dense(const double* L, const double* W, const double* R, double* M,
+                int chil, int Dl, int Dr, int d, int chir, hipStream_t s) {
+  return heff1_dense_b(L, W, R, M, chil, Dl, Dr, d, chir, 1, s);
 }
 
 // ---- elementwise helpers ---------------------------------------------------
@@ -423,6 +502,45 @@ int tnx_env_right(const double* R, const double* A, const double* W,
                   int precision, void* ws, size_t ws_bytes, tnx_stream_t stream) {
   return tnx::env_right(R, A, W, Rout, chil, Dl, Dr, d, chir, precision, ws,
                         ws_bytes, (hipStream_t)stream);
+}
+
+size_t tnx_heff1_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir, int batch) {
+  return tnx::heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch);
+}
+
+int tnx_heff1_matvec_batched(const double* L, const double* W, const double* R,
+                             const double* x, double* y, int chil, int Dl, int Dr,
+                             int d, int chir, int batch, int precision, void* ws,
+                             size_t ws_bytes, tnx_stream_t stream) {
+  const int64_t nx = (int64_t)chil * d * chir;
+  return tnx::heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, precision, batch, batch,
+                             nullptr, nx, nx, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int tnx_heff1_dense_batched(const double* L, const double* W, const double* R,
+                            double* M, int chil, int Dl, int Dr, int d, int chir,
+                            int batch, tnx_stream_t stream) {
+  return tnx::heff1_dense_b(L, W, R, M, chil, Dl, Dr, d, chir, batch, (hipStream_t)stream);
+}
+
+size_t tnx_env_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir, int batch) {
+  return tnx::env_ws_bytes_b(chil, Dl, Dr, d, chir, batch);
+}
+
+int tnx_env_left_batched(const double* L, const double* A, const double* W,
+                         double* Lout, int chil, int Dl, int Dr, int d, int chir,
+                         int batch, int precision, void* ws, size_t ws_bytes,
+                         tnx_stream_t stream) {
+  return tnx::env_left_b(L, A, W, Lout, chil, Dl, Dr, d, chir, precision, batch, ws, ws_bytes,
+                         (hipStream_t)stream);
+}
+
+int tnx_env_right_batched(const double* R, const double* A, const double* W,
+                          double* Rout, int chil, int Dl, int Dr, int d, int chir,
+                          int batch, int precision, void* ws, size_t ws_bytes,
+                          tnx_stream_t stream) {
+  return tnx::env_right_b(R, A, W, Rout, chil, Dl, Dr, d, chir, precision, batch, ws, ws_bytes,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -46,6 +46,9 @@ struct GemmArgs {
   int tiles_n;  // number of tiles along N
   int kc;       // K extent of one split (== K when not split)
   double* partial;  // split-K partial sums [batch][split][M][N], or null
+  int nb2;          // second batch level (1 = none)
+  int64_t sA2, sB2, sC2;
+  const int* bidx;  // launched -> logical batch index, or null
 };
 
 __device__ __forceinline__ double ld_any(const void* p, int t, int64_t i) {
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   const int wm0 = (wave / WGN) * WM;
   const int wn0 = (wave % WGN) * WN;
 
-  const int bat = blockIdx.y;
+  const int bat = g.bidx ? g.bidx[blockIdx.y] : (int)blockIdx.y;
   const int tile = blockIdx.x;
   const int split = blockIdx.z;
   const int kbeg = split * g.kc;
@@ -94,8 +97,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   const int m0 = (tile / g.tiles_n) * BM;
   const int n0 = (tile % g.tiles_n) * BN;
 
-  const char* Ab = (const char*)g.A + (size_t)bat * g.sA * (g.ta == 0 ? 8 : 4);
-  const char* Bb = (const char*)g.B + (size_t)bat * g.sB * (g.tb == 0 ? 8 : 4);
+  const int b1 = bat / g.nb2, b2 = bat % g.nb2;
+  const char* Ab = (const char*)g.A + ((size_t)b1 * g.sA + (size_t)b2 * g.sA2) * (g.ta == 0 ? 8 : 4);
+  const char* Bb = (const char*)g.B + ((size_t)b1 * g.sB + (size_t)b2 * g.sB2) * (g.tb == 0 ? 8 : 4);
 
   float ra[EA], rb[EB];
   double da[PREC == 0 ? EA : 1], db[PREC == 0 ? EB : 1];
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   //   f32 MFMAs: row = 4*lk + v ;  f64 MFMA: row = lk + 4*v ; column = lane%16
   auto row_of = [&](int v) { return (PREC == 0) ? (lk + 4 * v) : (lk * 4 + v); };
   if (g.partial) {
-    double* P = g.partial + ((size_t)bat * gridDim.z + split) * (size_t)g.M * g.N;
+    double* P = g.partial + ((size_t)blockIdx.y * gridDim.z + split) * (size_t)g.M * g.N;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
         }
     return;
   }
-  char* Cb = (char*)g.C + (size_t)bat * g.sC * (g.tc == 0 ? 8 : 4);
+  char* Cb = (char*)g.C + ((size_t)b1 * g.sC + (size_t)b2 * g.sC2) * (g.tc == 0 ? 8 : 4);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -277,15 +281,17 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
 // C = alpha * sum_z partial[bat][z] + beta * C   (fixed summation order)
 __global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, int M, int N,
                                      void* C, int tc, int64_t ldc, int64_t sC, double alpha,
-                                     double beta) {
-  const int bat = blockIdx.y;
+                                     double beta, int nb2, int64_t sC2, const int* bidx) {
+  const int bat = blockIdx.y;  // launched index (partials); logical below
+  const int lb = bidx ? bidx[bat] : bat;
+  const int b1 = lb / nb2, b2 = lb % nb2;
   const int64_t MN = (int64_t)M * N;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < MN;
        e += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
     for (int z = 0; z < splits; ++z) acc += P[((int64_t)bat * splits + z) * MN + e];
     const int64_t m = e / N, n = e % N;
-    const int64_t off = (int64_t)bat * sC + m * ldc + n;
+    const int64_t off = (int64_t)b1 * sC + (int64_t)b2 * sC2 + m * ldc + n;
     double val = alpha * acc;
     if (tc == 0) {
       double* c = reinterpret_cast<double*>(C) + off;
@@ -426,6 +432,9 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
   g.C = d.C; g.ldc = d.ldc; g.sC = d.sC; g.tc = (int)d.tc;
   g.alpha = d.alpha; g.beta = d.beta;
   g.tiles_n = 1;
+  g.nb2 = d.nb2 > 0 ? d.nb2 : 1;
+  g.sA2 = d.sA2; g.sB2 = d.sB2; g.sC2 = d.sC2;
+  g.bidx = d.bidx;
   const int splits = split_rule(d);
   g.kc = d.K;
   g.partial = nullptr;
@@ -445,13 +454,14 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
   if (ci < 0) {
     ci = heuristic_cfg<PREC>(d, nsplit);
     const int64_t work = (int64_t)d.M * d.N * d.K * d.batch;
-    if (autotune_on() && work >= ((int64_t)1 << 28)) {
+    if (autotune_on() && work >= ((int64_t)1 << 28) && !d.bidx) {
       // Time every config once on a scratch output (the caller's C is not
       // touched), keep the fastest. Runs once per shape key.
       GemmArgs gt = g;
       void* scratch = nullptr;
-      const size_t cbytes = ((size_t)(d.batch - 1) * d.sC + (size_t)(d.M - 1) * d.ldc + d.N) *
-                            (d.tc == DT::F64 ? 8 : 4);
+      const int nb2 = g.nb2;
+      const size_t cbytes = ((size_t)((d.batch - 1) / nb2) * d.sC + (size_t)((d.batch - 1) % nb2) * d.sC2 +
+                             (size_t)(d.M - 1) * d.ldc + d.N) * (d.tc == DT::F64 ? 8 : 4);
       if (!g.partial && hipMalloc(&scratch, cbytes) == hipSuccess) {
         gt.C = scratch;
         gt.beta = 0.0;
@@ -486,7 +496,8 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
     const int64_t MN = (int64_t)d.M * d.N;
     const int rb = (int)std::min<int64_t>((MN + 255) / 256, 2048);
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb, d.batch), dim3(256), 0, s, g.partial,
-                       nsplit, d.M, d.N, d.C, (int)d.tc, d.ldc, d.sC, d.alpha, d.beta);
+                       nsplit, d.M, d.N, d.C, (int)d.tc, d.ldc, d.sC, d.alpha, d.beta,
+                       g.nb2, g.sC2, g.bidx);
     TNX_LAUNCH_CHECK();
   }
   return TNX_OK;

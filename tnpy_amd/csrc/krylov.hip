@@ -42,6 +42,17 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Problem map of a batched launch: launched problem i = blockIdx.y works on
+// logical problem idx[i] (or i when idx is null), whose work arrays sit at
+// +problem * sp doubles from the base pointers. One problem: idx null, sp 0.
+struct PMap {
+  const int* idx;
+  int64_t sp;
+  __device__ __forceinline__ int pid() const { return idx ? idx[blockIdx.y] : (int)blockIdx.y; }
+  __device__ __forceinline__ int64_t off() const { return (int64_t)pid() * sp; }
+};
+static const PMap kOne{nullptr, 0};
+
 constexpr int kMaxBasis = 64;        // Lanczos vectors kept before restart
 constexpr int kPartStride = kMaxBasis + 2;
 constexpr int kCgsBlocks = 512;      // blocks of a CGS pass (partials rows)
@@ -56,7 +67,12 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict_
                                                        double* __restrict__ w,
                                                        const double* __restrict__ hsub,
                                                        double* __restrict__ partials,
-                                                       int64_t N) {
+                                                       int64_t N, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    V += o; w += o; partials += o;
+    if (hsub) hsub += o;
+  }
   __shared__ double hs[JMAX];
   __shared__ double red[4][JMAX + 1];
   if (threadIdx.x < JMAX) hs[threadIdx.x] = (hsub && (int)threadIdx.x < J) ? hsub[threadIdx.x] : 0.0;
@@ -104,7 +120,11 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict_
 // out[j] = sum over blocks of partials[blk][j], j <= J. One workgroup per
 // j; a fixed strided split and a fixed tree, so the sum is deterministic.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* partials, int nblk,
-                                                              int J, double* out) {
+                                                              int J, double* out, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    partials += o; out += o;
+  }
   __shared__ double red[4];
   const int j = blockIdx.x;
   double s = 0.0;
@@ -116,13 +136,15 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* part
 }
 
 static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double* hsub,
-                    bool dots, double* partials, double* out, int64_t N, hipStream_t s) {
-  const int nblk = (int)std::min<int64_t>(kCgsBlocks, (N + 255) / 256);
+                    bool dots, double* partials, double* out, int64_t N, hipStream_t s,
+                    PMap pm = kOne, int count = 1) {
+  const int nblk = (int)std::min<int64_t>(count > 1 ? std::max(kCgsBlocks / count, 8) : kCgsBlocks,
+                                          (N + 255) / 256);
   auto go = [&](auto jtag, auto dtag) {
     constexpr int JM = decltype(jtag)::value;
     constexpr bool DT_ = decltype(dtag)::value;
-    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_>), dim3(nblk), dim3(256), 0, s, V, ldv, J, w,
-                       hsub, partials, N);
+    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_>), dim3(nblk, count), dim3(256), 0, s, V, ldv, J,
+                       w, hsub, partials, N, pm);
   };
   using T8 = std::integral_constant<int, 8>;
   using T16 = std::integral_constant<int, 16>;
@@ -136,14 +158,20 @@ static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double
   else if (J <= 32) { if (dots) go(T32{}, BT{}); else go(T32{}, BF{}); }
   else { if (dots) go(T64{}, BT{}); else go(T64{}, BF{}); }
   TNX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(J + 1), dim3(256), 0, s, partials, nblk, J, out);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(J + 1, count), dim3(256), 0, s, partials, nblk,
+                     J, out, pm);
   TNX_LAUNCH_CHECK();
   return TNX_OK;
 }
 
 // out[i] = sum_j c[j] V[j][i]   (Ritz vector from the basis)
 __global__ void vcomb_kernel(const double* __restrict__ V, int64_t ldv, int J,
-                             const double* __restrict__ c, double* __restrict__ out, int64_t N) {
+                             const double* __restrict__ c, double* __restrict__ out, int64_t N,
+                             PMap pm) {
+  {
+    const int64_t o = pm.off();
+    V += o; c += o; out += o;
+  }
   __shared__ double cs[kMaxBasis + 1];
   if ((int)threadIdx.x < J) cs[threadIdx.x] = c[threadIdx.x];
   __syncthreads();
@@ -157,7 +185,11 @@ __global__ void vcomb_kernel(const double* __restrict__ V, int64_t ldv, int J,
 
 // out = x / sqrt(*nrm2)  (unit vector; zero if the norm underflows)
 __global__ void normalize_kernel(const double* __restrict__ x, const double* __restrict__ nrm2,
-                                 double* __restrict__ out, int64_t N) {
+                                 double* __restrict__ out, int64_t N, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    x += o; nrm2 += o; out += o;
+  }
   const double n2 = *nrm2;
   const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -165,10 +197,18 @@ __global__ void normalize_kernel(const double* __restrict__ x, const double* __r
     out[i] = x[i] * inv;
 }
 
-// out = (*a) * x + (*b) * z, scalars read from device memory
+// out = (*a) * x + (*b) * z, scalars read from device memory. Batched:
+// a, x, b, z move with the problem map, out by so doubles per problem.
 __global__ void axpbz_dev_kernel(const double* __restrict__ a, const double* __restrict__ x,
                                  const double* __restrict__ b, const double* __restrict__ z,
-                                 double* __restrict__ out, int64_t N) {
+                                 double* __restrict__ out, int64_t N, PMap pm, int64_t so) {
+  {
+    const int64_t o = pm.off();
+    a += o; x += o;
+    if (b) b += o;
+    if (z) z += o;
+    out += (int64_t)pm.pid() * so;
+  }
   const double av = *a, bv = b ? *b : 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -186,7 +226,20 @@ __global__ void scal_kernel(int64_t n, double a, double* __restrict__ x) {
     x[i] *= a;
 }
 
-static int vec_blocks(int64_t N) { return (int)std::min<int64_t>((N + 255) / 256, 4096); }
+static int vec_blocks(int64_t N, int count = 1) {
+  return (int)std::min<int64_t>((N + 255) / 256, count > 1 ? std::max(4096 / count, 8) : 4096);
+}
+
+// out_p = in_p for the mapped problems: in moves with the map (sp), out by so.
+__global__ void copy_map_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                int64_t N, PMap pm, int64_t sin_, int64_t so) {
+  const int64_t p = pm.pid();
+  in += p * sin_;
+  out += p * so;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
 
 // ---------------------------------------------------------------------------
 // Symmetric tridiagonal lowest eigenpair, one wavefront.
@@ -214,7 +267,12 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
                                                             const double* __restrict__ bd, int n,
                                                             const double* __restrict__ bres,
                                                             double* __restrict__ s,
-                                                            double* __restrict__ st) {
+                                                            double* __restrict__ st, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    ad += o; bd += o; s += o; st += o;
+    if (bres) bres += o;
+  }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
   __shared__ double u0[kTriMax], u1[kTriMax], u2[kTriMax], z[kTriMax];
   __shared__ double shr[4];
@@ -334,7 +392,11 @@ __global__ __launch_bounds__(256) void sytrd_kernel(double* __restrict__ A, int 
                                                    double* __restrict__ Vst,
                                                    double* __restrict__ tau,
                                                    double* __restrict__ dg,
-                                                   double* __restrict__ eg) {
+                                                   double* __restrict__ eg, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    A += o; Vst += o; tau += o; dg += o; eg += o;
+  }
   __shared__ double v[kTriMax], p[kTriMax];
   __shared__ double red[8];
   const int t = threadIdx.x;
@@ -390,7 +452,12 @@ __global__ __launch_bounds__(256) void sytrd_kernel(double* __restrict__ A, int 
 __global__ __launch_bounds__(256) void ormtr_kernel(const double* __restrict__ Vst,
                                                    const double* __restrict__ tau, int n,
                                                    const double* __restrict__ zin,
-                                                   double* __restrict__ x) {
+                                                   double* __restrict__ x, PMap pm, int64_t so) {
+  {
+    const int64_t o = pm.off();
+    Vst += o; tau += o; zin += o;
+    x += (int64_t)pm.pid() * so;
+  }
   __shared__ double xs[kTriMax];
   __shared__ double red[8];
   const int t = threadIdx.x;
@@ -437,12 +504,13 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
     }
     return TNX_OK;
   }
-  hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg);
+  hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg, kOne);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st);
+                     (const double*)nullptr, z, st, kOne);
   TNX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec);
+  hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec, kOne,
+                     (int64_t)0);
   TNX_LAUNCH_CHECK();
   if (eval_host) {
     TNX_HIP(hipMemcpyAsync(eval_host, st, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -463,7 +531,12 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
 __global__ __launch_bounds__(64) void tridiag_all_kernel(const double* __restrict__ ad,
                                                          const double* __restrict__ bd, int n,
                                                          double* __restrict__ w,
-                                                         double* __restrict__ S, int ldS) {
+                                                         double* __restrict__ S, int ldS,
+                                                         PMap pm) {
+  {
+    const int64_t o = pm.off();
+    ad += o; bd += o; w += o; S += o;
+  }
   __shared__ double d[64], e[64];
   __shared__ double Z[64][65];
   const int k = threadIdx.x;
@@ -613,7 +686,11 @@ static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
 // Finish a Lanczos column j: alpha[j] = h1[j] + h2[j], beta[j] = |w|, and
 // the same entries in the dense projected matrix T.
 __global__ void lanczos_finish_kernel(const double* h1, const double* h2, const double* h3,
-                                      int j, double* alpha, double* beta, double* T) {
+                                      int j, double* alpha, double* beta, double* T, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    h1 += o; h2 += o; h3 += o; alpha += o; beta += o; T += o;
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     const double a = h1[j] + h2[j];
     const double b = sqrt(fmax(h3[j + 1], 0.0));
@@ -630,7 +707,11 @@ __global__ void lanczos_finish_kernel(const double* h1, const double* h2, const 
 // H y = theta y + (beta_j s_j) V[j+1] for the lowest Ritz pair:
 // scal[0] = theta, scal[1] = beta_j * s_j
 __global__ void ritz_scalars_kernel(const double* st, const double* s, const double* beta, int j,
-                                    double* scal) {
+                                    double* scal, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    st += o; s += o; beta += o; scal += o;
+  }
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     scal[0] = st[0];
     scal[1] = beta[j] * s[j];
@@ -652,7 +733,13 @@ __global__ __launch_bounds__(64) void tr_restart_kernel(const double* __restrict
                                                         int k, int J, double* __restrict__ Cm,
                                                         double* __restrict__ alpha,
                                                         double* __restrict__ beta,
-                                                        double* __restrict__ T, int* kp_out) {
+                                                        double* __restrict__ T, int* kp_out,
+                                                        PMap pm) {
+  {
+    const int64_t o = pm.off();
+    theta += o; S += o; beta_in += o; Cm += o; alpha += o; beta += o; T += o;
+    kp_out = reinterpret_cast<int*>(reinterpret_cast<double*>(kp_out) + o);
+  }
   __shared__ double q[kMaxBasis][kMaxBasis];
   __shared__ double a[kMaxBasis], b[kMaxBasis];
   const int l = threadIdx.x;  // component index of the k-vectors
@@ -679,11 +766,33 @@ __global__ __launch_bounds__(64) void tr_restart_kernel(const double* __restrict
     if (l == 0) a[i] = ai;
     const double bn = sqrt(wave_sum(w * w));
     kp = i + 1;
-    if (i == k - 1 || !(bn > 1e-13 * fmax(tmax, 1e-300))) break;
-    if (l == 0) b[i] = bn;
-    qprev = qi;
-    bprev = bn;
-    qi = w / bn;
+    if (i == k - 1) break;
+    if (bn > 1e-13 * fmax(tmax, 1e-300)) {
+      if (l == 0) b[i] = bn;
+      qprev = qi;
+      bprev = bn;
+      qi = w / bn;
+    } else {
+      // span(q[0..i]) is invariant under diag(theta): continue in its
+      // complement from the unit vector e_t with the largest component
+      // there (zero coupling b[i]), so exactly k vectors are kept.
+      double best = -1.0, qb = 0.0;
+      for (int t = 0; t < k; ++t) {
+        double u = (l == t) ? 1.0 : 0.0;
+        for (int pass = 0; pass < 2; ++pass)
+          for (int r = 0; r <= i; ++r) {
+            const double h = wave_sum(q[r][l] * u);
+            u -= h * q[r][l];
+          }
+        if (l >= k) u = 0.0;
+        const double un = sqrt(wave_sum(u * u));
+        if (un > best) { best = un; qb = u / un; }
+      }
+      if (l == 0) b[i] = 0.0;
+      qprev = 0.0;
+      bprev = 0.0;
+      qi = qb;
+    }
     q[i + 1][l] = qi;
   }
   __syncthreads();
@@ -717,7 +826,11 @@ __global__ __launch_bounds__(64) void tr_restart_kernel(const double* __restrict
 // two-vector basis [y, r/rho].
 __global__ void outer_scalars_kernel(const double* h1, const double* h2, const double* h3,
                                      double* alpha, double* beta, double* scal, double* st,
-                                     double* T) {
+                                     double* T, PMap pm) {
+  {
+    const int64_t o = pm.off();
+    h1 += o; h2 += o; h3 += o; alpha += o; beta += o; scal += o; st += o; T += o;
+  }
   const double th = h1[0] + h2[0];
   const double rho = sqrt(fmax(h3[1], 0.0));
   for (int e = threadIdx.x; e < kMaxBasis * kMaxBasis; e += blockDim.x) {
@@ -770,7 +883,7 @@ static int outer_step(EigCtx& c, double* theta, double* rho) {
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h1, true, k.partials, k.h2, N, c.s));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h2, false, k.partials, k.h3, N, c.s));
   hipLaunchKernelGGL(outer_scalars_kernel, dim3(1), dim3(256), 0, c.s, k.h1, k.h2, k.h3,
-                     k.alpha, k.beta, k.scal, k.st, k.T);
+                     k.alpha, k.beta, k.scal, k.st, k.T, kOne);
   TNX_LAUNCH_CHECK();
   double h[8];
   TNX_TRY(read_status(c, h, 8));
@@ -835,7 +948,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
 
   // y = v0 / |v0|
   TNX_TRY(cgs_pass(v0, N, 0, const_cast<double*>(v0), nullptr, false, k.partials, k.h3, N, s));
-  hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, v0, k.h3, k.y, N);
+  hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, v0, k.h3, k.y, N, kOne);
   TNX_LAUNCH_CHECK();
 
   double theta = 0.0, rho = 0.0, anorm = 0.0;
@@ -864,7 +977,8 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
     if (c.matvecs >= max_mv) break;
     // V[0] = y, V[1] = r / rho
     TNX_TRY(copy_f64(k.y, k.V, N, s));
-    hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, k.V + N, N);
+    hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, k.V + N, N,
+                       kOne);
     TNX_LAUNCH_CHECK();
     // Inner Lanczos cycle in `prec`, from column j = 1.
     // Inner target. FP64: the final tolerance. Low precision: the first
@@ -889,14 +1003,14 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
       TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
       hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
-                         k.alpha, k.beta, k.T);
+                         k.alpha, k.beta, k.T, kOne);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                         k.beta + j, k.s, k.st);
+                         k.beta + j, k.s, k.st, kOne);
       TNX_LAUNCH_CHECK();
       // V[j+1] = w / beta[j]
       hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
-                         k.V + (int64_t)(j + 1) * N, N);
+                         k.V + (int64_t)(j + 1) * N, N, kOne);
       TNX_LAUNCH_CHECK();
       double h[4];
       TNX_TRY(read_status(c, h, 3));
@@ -904,20 +1018,23 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       const double est = h[1];
       anorm = std::max(anorm, h[2]);
       const bool exhausted = (J >= N);
-      if (est <= inner_tol || exhausted || c.matvecs >= max_mv) {
+      // FP64 cycles test against the current ||H|| estimate (it grows with
+      // the Ritz values seen); low-precision cycles against their fixed target
+      const double itol = mixed ? inner_tol : tol * anorm;
+      if (est <= itol || exhausted || c.matvecs >= max_mv) {
         // Ritz vector y = V[0..j] s
-        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N);
+        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N, kOne);
         TNX_LAUNCH_CHECK();
         inner_done = true;
         if (!mixed && (est <= tol * anorm || exhausted)) {
           // FP64 Lanczos: H y = theta y + beta_j s_j V[j+1]
           hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, k.beta, j,
-                             k.scal);
+                             k.scal, kOne);
           TNX_LAUNCH_CHECK();
           if (hvec) {
             hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal, k.y,
                                exhausted ? nullptr : k.scal + 1, k.V + (int64_t)(j + 1) * N,
-                               hvec, N);
+                               hvec, N, kOne, (int64_t)0);
             TNX_LAUNCH_CHECK();
           }
           theta = theta_in;
@@ -928,16 +1045,16 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         // Thick restart keeping the `keep` lowest Ritz pairs in a basis
         // where the projected matrix stays tridiagonal.
         hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                           k.theta, k.S, kMaxBasis);
+                           k.theta, k.S, kMaxBasis, kOne);
         TNX_LAUNCH_CHECK();
         int* kp_dev = reinterpret_cast<int*>(k.st + 8);
         hipLaunchKernelGGL(tr_restart_kernel, dim3(1), dim3(64), 0, s, k.theta, k.S, k.beta, j,
-                           keep, J, k.Cm, k.alpha, k.beta, k.T, kp_dev);
+                           keep, J, k.Cm, k.alpha, k.beta, k.T, kp_dev, kOne);
         TNX_LAUNCH_CHECK();
         int kp = 0;
         TNX_HIP(hipMemcpyAsync(&kp, kp_dev, sizeof(int), hipMemcpyDeviceToHost, s));
         TNX_HIP(hipStreamSynchronize(s));
-        TNX_CHECK_ARG(kp >= 1 && kp <= keep, "eigsh: restart kept %d vectors", kp);
+        TNX_CHECK_ARG(kp == keep, "eigsh: restart kept %d of %d vectors", kp, keep);
         GemmDesc g{};
         g.M = kp; g.N = (int)N; g.K = J; g.batch = 1;
         g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = 0; g.a_kcontig = false;
@@ -965,7 +1082,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
     if (hvec) {
       // H y = r + theta y  (scal[3] = 1, scal[0] = theta)
       hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal + 3, k.w, k.scal,
-                         k.y, hvec, N);
+                         k.y, hvec, N, kOne, (int64_t)0);
       TNX_LAUNCH_CHECK();
     }
   } else {
@@ -981,6 +1098,305 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
   info->restarts = restarts;
   info->converged = converged ? 1 : 0;
   return converged ? TNX_OK : TNX_ERR_NOCONV;
+}
+
+// ---------------------------------------------------------------------------
+// Batched eigensolves: `batch` independent one-site problems of one shape
+// (stacks of L, W, R, v0; outputs vec, hvec stacked). FP64 thick-restart
+// Lanczos in lockstep: every problem is at the same Lanczos step (restarts
+// keep exactly `keep` vectors), each problem's Krylov scalars are its own,
+// and a problem leaves the launched set (device index list) once its
+// residual estimate meets tol * ||H||_est. One launch per kernel per step
+// covers all problems still iterating.
+// ---------------------------------------------------------------------------
+int heff1_matvec_b(const double* L, const double* W, const double* R, const double* x, double* y,
+                   int chil, int Dl, int Dr, int d, int chir, int prec, int batch, int count,
+                   const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
+                   hipStream_t s);
+size_t heff1_ws_bytes_b(int chil, int Dl, int Dr, int d, int chir, int batch);
+
+// out[i * nst + t] = st[first + t] of launched problem i
+__global__ void gather_status_kernel(const double* __restrict__ st, int first, int nst,
+                                     double* __restrict__ out, PMap pm) {
+  const int t = threadIdx.x;
+  if (t < nst) out[(int64_t)blockIdx.y * nst + t] = st[pm.off() + first + t];
+}
+
+static size_t eigsh_b_problem_bytes(size_t N, int m) { return align256(eigsh_ws_bytes_n(N, 0, m)); }
+
+size_t eigsh_b_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis, int batch) {
+  const int m = std::max(3, std::min(max_basis, kMaxBasis));
+  const size_t N = (size_t)chil * d * chir;
+  return align256(heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch)) +
+         2 * align256((size_t)batch * sizeof(int)) + align256((size_t)batch * 4 * 8) +
+         eigsh_b_problem_bytes(N, m) * (size_t)batch + 256;
+}
+
+int eigsh_heff1_b(const double* L, const double* W, const double* R, const double* v0,
+                  double* vec, double* hvec, int chil, int Dl, int Dr, int d, int chir,
+                  int batch, const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
+                  size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(opts && info, "eigsh_batched: null opts/info");
+  TNX_CHECK_ARG(L && W && R && v0 && vec && hvec, "eigsh_batched: null pointer");
+  TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dr > 0 && d > 0 && chir > 0, "eigsh_batched: bad dims");
+  TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "eigsh_batched: bad batch %d", batch);
+  TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh_batched: negative tol");
+  if (opts->precision != TNX_F64) {
+    set_error("eigsh_batched: only FP64 (precision 0) is supported, got %d", opts->precision);
+    return TNX_ERR_UNSUPPORTED;
+  }
+  const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
+  const int64_t N = (int64_t)chil * d * chir;
+  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_b_ws_bytes(chil, Dl, Dr, d, chir, m, batch),
+                "eigsh_batched: workspace too small");
+  const int max_mv = opts->max_matvecs > 0 ? opts->max_matvecs : 10000;
+  const double tol = opts->tol;
+  const int mb = (int)std::min<int64_t>(m, std::max<int64_t>(N, 1));
+  const int keep = keep_of(mb, opts->keep);
+
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  void* mv = p;
+  const size_t mvb = heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch);
+  p += align256(mvb);
+  int* idx = (int*)p; p += align256((size_t)batch * sizeof(int));
+  int* fidx = (int*)p; p += align256((size_t)batch * sizeof(int));
+  double* stat = (double*)p; p += align256((size_t)batch * 4 * 8);
+  const size_t P = eigsh_b_problem_bytes((size_t)N, m);
+  const int64_t sp = (int64_t)(P / 8);
+  KWork k;
+  carve(k, p, m, (size_t)N, 0);  // problem 0; problem q at + q * sp doubles
+
+  std::vector<int> act(batch), fin;
+  for (int b = 0; b < batch; ++b) act[b] = b;
+  std::vector<double> anorm(batch, 0.0), hs((size_t)batch * 4);
+  for (int b = 0; b < batch; ++b) info[b] = tnx_eigsh_info_t{0.0, 0.0, 0.0, 0, 0, 0, 0};
+  int matvecs = 0, restarts = 0;
+  bool any_fail = false;
+
+  auto upload = [&](int* dst, const std::vector<int>& v) -> int {
+    TNX_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    return TNX_OK;
+  };
+  auto status = [&](int first, int nst, int cnt) -> int {
+    hipLaunchKernelGGL(gather_status_kernel, dim3(1, cnt), dim3(64), 0, s, k.st, first, nst, stat,
+                       PMap{idx, sp});
+    TNX_LAUNCH_CHECK();
+    TNX_HIP(hipMemcpyAsync(hs.data(), stat, sizeof(double) * nst * cnt, hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipStreamSynchronize(s));
+    return TNX_OK;
+  };
+  auto matvec = [&](const double* x, double* y, int cnt) -> int {
+    matvecs++;
+    return heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, TNX_F64, batch, cnt, idx, sp, sp,
+                          mv, mvb, s);
+  };
+
+  TNX_TRY(upload(idx, act));
+  int cnt = batch;
+  PMap pm{idx, sp};
+  // y = v0 / |v0|
+  hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, v0, k.y, N,
+                     pm, N, sp);
+  TNX_LAUNCH_CHECK();
+  TNX_TRY(cgs_pass(k.y, N, 0, k.y, nullptr, false, k.partials, k.h3, N, s, pm, cnt));
+  hipLaunchKernelGGL(normalize_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.y, k.h3,
+                     k.y, N, pm);
+  TNX_LAUNCH_CHECK();
+  // FP64 step on the start vectors: theta = y.Hy, r = Hy - theta y
+  TNX_TRY(matvec(k.y, k.w, cnt));
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, nullptr, true, k.partials, k.h1, N, s, pm, cnt));
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h1, true, k.partials, k.h2, N, s, pm, cnt));
+  TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
+  hipLaunchKernelGGL(outer_scalars_kernel, dim3(1, cnt), dim3(256), 0, s, k.h1, k.h2, k.h3,
+                     k.alpha, k.beta, k.scal, k.st, k.T, pm);
+  TNX_LAUNCH_CHECK();
+  TNX_TRY(status(4, 2, cnt));
+  {
+    std::vector<int> rest;
+    fin.clear();
+    for (int i = 0; i < cnt; ++i) {
+      const int b = act[i];
+      const double th = hs[2 * i], rho = hs[2 * i + 1];
+      anorm[b] = std::max(anorm[b], std::fabs(th));
+      info[b].eval = th;
+      info[b].resnorm = rho;
+      info[b].matvecs = info[b].matvecs_f64 = matvecs;
+      if (rho <= tol * anorm[b] || rho == 0.0 || N == 1) {
+        fin.push_back(b);
+        info[b].converged = 1;
+      } else {
+        rest.push_back(b);
+      }
+    }
+    if (!fin.empty()) {
+      // converged start vectors: vec = y, hvec = r + theta y
+      TNX_TRY(upload(fidx, fin));
+      const int fc = (int)fin.size();
+      PMap pf{fidx, sp};
+      hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(N, fc), fc), dim3(256), 0, s, k.y, vec,
+                         N, pf, sp, N);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(axpbz_dev_kernel, dim3(vec_blocks(N, fc), fc), dim3(256), 0, s,
+                         k.scal + 3, k.w, k.scal, k.y, hvec, N, pf, N);
+      TNX_LAUNCH_CHECK();
+    }
+    act.swap(rest);
+  }
+  cnt = (int)act.size();
+  if (cnt > 0) {
+    TNX_TRY(upload(idx, act));
+    // V[0] = y, V[1] = r / rho
+    hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.y, k.V,
+                       N, pm, sp, sp);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(normalize_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.w,
+                       k.h3 + 1, k.V + N, N, pm);
+    TNX_LAUNCH_CHECK();
+  }
+  int j = 1;
+  while (cnt > 0) {
+    const int nb = vec_blocks(N, cnt);
+    double* vj = k.V + (int64_t)j * N;
+    TNX_TRY(matvec(vj, k.w, cnt));
+    const int J = j + 1;
+    TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s, pm, cnt));
+    TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s, pm, cnt));
+    TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
+    hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1, cnt), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
+                       k.alpha, k.beta, k.T, pm);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
+                       k.beta + j, k.s, k.st, pm);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(normalize_kernel, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
+                       k.V + (int64_t)(j + 1) * N, N, pm);
+    TNX_LAUNCH_CHECK();
+    TNX_TRY(status(0, 3, cnt));
+    const bool exhausted = (J >= N);
+    const bool limit = matvecs >= max_mv;
+    std::vector<int> rest;
+    fin.clear();
+    for (int i = 0; i < cnt; ++i) {
+      const int b = act[i];
+      const double th = hs[3 * i], est = hs[3 * i + 1];
+      anorm[b] = std::max(anorm[b], hs[3 * i + 2]);
+      const bool conv = est <= tol * anorm[b] || exhausted;
+      if (conv || limit) {
+        fin.push_back(b);
+        info[b].eval = th;
+        info[b].resnorm = est;
+        info[b].matvecs = info[b].matvecs_f64 = matvecs;
+        info[b].restarts = restarts;
+        info[b].converged = conv ? 1 : 0;
+        any_fail |= !conv;
+      } else {
+        rest.push_back(b);
+      }
+    }
+    if (!fin.empty()) {
+      // Ritz vector y = V s; H y = theta y + beta_j s_j V[j+1]
+      TNX_TRY(upload(fidx, fin));
+      const int fc = (int)fin.size();
+      const int nf = vec_blocks(N, fc);
+      PMap pf{fidx, sp};
+      hipLaunchKernelGGL(vcomb_kernel, dim3(nf, fc), dim3(256), 0, s, k.V, N, J, k.s, k.y, N, pf);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1, fc), dim3(1), 0, s, k.st, k.s, k.beta, j,
+                         k.scal, pf);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nf, fc), dim3(256), 0, s, k.scal, k.y,
+                         exhausted ? nullptr : k.scal + 1, k.V + (int64_t)(j + 1) * N, hvec, N,
+                         pf, N);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(copy_map_kernel, dim3(nf, fc), dim3(256), 0, s, k.y, vec, N, pf, sp, N);
+      TNX_LAUNCH_CHECK();
+      act.swap(rest);
+      cnt = (int)act.size();
+      if (cnt == 0) break;
+      TNX_TRY(upload(idx, act));
+    }
+    if (J >= mb) {
+      hipLaunchKernelGGL(tridiag_all_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
+                         k.theta, k.S, kMaxBasis, pm);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(tr_restart_kernel, dim3(1, cnt), dim3(64), 0, s, k.theta, k.S, k.beta, j,
+                         keep, J, k.Cm, k.alpha, k.beta, k.T,
+                         reinterpret_cast<int*>(k.st + 8), pm);
+      TNX_LAUNCH_CHECK();
+      // Y = C^T V (keep x N per problem), then V[0..keep) = Y, V[keep] = V[j+1]
+      GemmDesc g{};
+      g.M = keep; g.N = (int)N; g.K = J; g.batch = cnt; g.bidx = idx;
+      g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = sp; g.a_kcontig = false;
+      g.B = k.V; g.tb = DT::F64; g.ldb = N; g.sB = sp; g.b_ncontig = true;
+      g.C = k.Y; g.tc = DT::F64; g.ldc = N; g.sC = sp;
+      g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+      TNX_TRY(gemm(g, s));
+      hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks((int64_t)keep * N, cnt), cnt),
+                         dim3(256), 0, s, k.Y, k.V, (int64_t)keep * N, pm, sp, sp);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(copy_map_kernel, dim3(nb, cnt), dim3(256), 0, s,
+                         k.V + (int64_t)(j + 1) * N, k.V + (int64_t)keep * N, N, pm, sp, sp);
+      TNX_LAUNCH_CHECK();
+      restarts++;
+      j = keep;
+    } else {
+      j++;
+    }
+  }
+  TNX_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < batch; ++b) info[b].anorm = anorm[b];
+  if (any_fail) {
+    set_error("eigsh_batched: matvec limit %d reached before convergence", max_mv);
+    return TNX_ERR_NOCONV;
+  }
+  return TNX_OK;
+}
+
+// Lowest eigenpairs of `batch` dense symmetric n x n matrices (stacked).
+size_t syev_b_ws_bytes(int n, int batch) {
+  return align256((size_t)batch * 8) + align256(syev_ws_bytes(n)) * (size_t)batch + 256;
+}
+
+int syev_lowest_b(const double* M, int n, int batch, double* evec, double* eval_host, void* ws,
+                  size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(n >= 1 && n <= kTriMax, "syev_batched: n=%d outside [1, %d]", n, kTriMax);
+  TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "syev_batched: bad batch %d", batch);
+  TNX_CHECK_ARG(M && evec && eval_host, "syev_batched: null pointer");
+  TNX_CHECK_ARG(ws && ws_bytes >= syev_b_ws_bytes(n, batch), "syev_batched: workspace too small");
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* ev = (double*)p; p += align256((size_t)batch * 8);
+  const int64_t nn = (int64_t)n * n;
+  if (n == 1) {
+    TNX_TRY(fill_f64(evec, batch, 1.0, s));
+    TNX_HIP(hipMemcpyAsync(eval_host, M, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipStreamSynchronize(s));
+    return TNX_OK;
+  }
+  const int64_t sp = (int64_t)(align256(syev_ws_bytes(n)) / 8);
+  double* A = (double*)p;
+  double* Vst = A + nn;
+  double* tau = Vst + nn;
+  double* dg = tau + n;
+  double* eg = dg + n;
+  double* z = eg + n;
+  double* st = z + n + 8;
+  const PMap pm{nullptr, sp};
+  hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(nn, batch), batch), dim3(256), 0, s, M, A,
+                     nn, pm, nn, sp);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sytrd_kernel, dim3(1, batch), dim3(256), 0, s, A, n, Vst, tau, dg, eg, pm);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, batch), dim3(64), 0, s, dg, eg, n,
+                     (const double*)nullptr, z, st, pm);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ormtr_kernel, dim3(1, batch), dim3(256), 0, s, Vst, tau, n, z, evec, pm,
+                     (int64_t)n);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gather_status_kernel, dim3(1, batch), dim3(64), 0, s, st, 0, 1, ev, pm);
+  TNX_LAUNCH_CHECK();
+  TNX_HIP(hipMemcpyAsync(eval_host, ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+  TNX_HIP(hipStreamSynchronize(s));
+  return TNX_OK;
 }
 
 }  // namespace tnx
@@ -1062,6 +1478,30 @@ int tnx_dot(int64_t n, const double* x, const double* y, double* out_host, void*
   TNX_HIP(hipMemcpyAsync(out_host, out, sizeof(double), hipMemcpyDeviceToHost, s));
   TNX_HIP(hipStreamSynchronize(s));
   return TNX_OK;
+}
+
+size_t tnx_syev_lowest_batched_workspace_bytes(int n, int batch) {
+  return tnx::syev_b_ws_bytes(n, batch);
+}
+
+int tnx_syev_lowest_batched(const double* M, int n, int batch, double* evec,
+                            double* eval_host, void* ws, size_t ws_bytes,
+                            tnx_stream_t stream) {
+  return tnx::syev_lowest_b(M, n, batch, evec, eval_host, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t tnx_eigsh_heff1_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int chir,
+                                               int max_basis, int batch) {
+  return tnx::eigsh_b_ws_bytes(chil, Dl, Dr, d, chir, max_basis, batch);
+}
+
+int tnx_eigsh_heff1_batched(const double* L, const double* W, const double* R,
+                            const double* v0, double* vec, double* hvec, int chil,
+                            int Dl, int Dr, int d, int chir, int batch,
+                            const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
+                            void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::eigsh_heff1_b(L, W, R, v0, vec, hvec, chil, Dl, Dr, d, chir, batch, opts, info, ws,
+                            ws_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

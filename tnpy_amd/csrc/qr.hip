@@ -396,6 +396,161 @@ int gauge_left(double* A, const double* prev, double* prev_out, int chil, int d,
   return gemm(g, s);
 }
 
+// ---- batched small QR ----------------------------------------------------------
+// Householder QR of `batch` small m x n matrices (m >= n), one workgroup per
+// matrix held in LDS (row stride n + 1). trans = 0: matrix p is A_p, m x n
+// row-major. trans = 1: matrix p is A_p^T, where A_p is n x m row-major (the
+// leftward gauge move: A = (Q R)^T). On exit A_p holds the thin Q (Q^T when
+// trans) and R_p = R (n x n, upper triangular). Same reflector convention as
+// the blocked path (beta = -sign(alpha) |x|).
+constexpr int kQrSmallBig = 16640;   // m * (n + 1) doubles: 130 KiB of LDS
+constexpr int kQrSmallLittle = 4352; // 34 KiB
+
+__device__ __forceinline__ double qs_block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wsum(v);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+template <int MAXE>
+__global__ __launch_bounds__(256) void qr_small_kernel(double* __restrict__ A, int m, int n,
+                                                       int trans, double* __restrict__ R) {
+  __shared__ double S[MAXE];
+  __shared__ double tau_s[256];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ld = n + 1;
+  const int64_t mn = (int64_t)m * n;
+  double* Ap = A + (int64_t)blockIdx.x * mn;
+  for (int e = tid; e < mn; e += 256) {
+    int i, c;
+    if (trans) { c = e / m; i = e - c * m; } else { i = e / n; c = e - i * n; }
+    S[i * ld + c] = Ap[e];
+  }
+  __syncthreads();
+  // ---- factorisation
+  for (int j = 0; j < n; ++j) {
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += 256) part += S[i * ld + j] * S[i * ld + j];
+    const double sigma = qs_block_sum(part, red);
+    const double alpha = S[j * ld + j];
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (sigma != 0.0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < m; i += 256) S[i * ld + j] *= scale;
+    if (tid == 0) { S[j * ld + j] = beta; tau_s[j] = tj; }
+    __syncthreads();
+    if (tj != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += 4) {
+        double acc = 0.0;
+        for (int i = j + lane; i < m; i += 64) acc += (i == j ? 1.0 : S[i * ld + j]) * S[i * ld + c];
+        const double w = tj * wsum(acc);
+        for (int i = j + lane; i < m; i += 64) S[i * ld + c] -= (i == j ? 1.0 : S[i * ld + j]) * w;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- R
+  double* Rp = R + (int64_t)blockIdx.x * n * n;
+  for (int e = tid; e < n * n; e += 256) {
+    const int i = e / n, c = e - i * n;
+    Rp[e] = (c >= i) ? S[i * ld + c] : 0.0;
+  }
+  __syncthreads();
+  // ---- thin Q = H_0 ... H_{n-1} [I; 0], formed in place from the last column
+  for (int j = n - 1; j >= 0; --j) {
+    const double tj = tau_s[j];
+    if (tj != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += 4) {
+        double acc = 0.0;
+        for (int i = j + lane; i < m; i += 64) acc += (i == j ? 1.0 : S[i * ld + j]) * S[i * ld + c];
+        const double w = tj * wsum(acc);
+        for (int i = j + lane; i < m; i += 64) S[i * ld + c] -= (i == j ? 1.0 : S[i * ld + j]) * w;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += 256)
+      S[i * ld + j] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * S[i * ld + j]);
+    __syncthreads();
+  }
+  for (int e = tid; e < mn; e += 256) {
+    int i, c;
+    if (trans) { c = e / m; i = e - c * m; } else { i = e / n; c = e - i * n; }
+    Ap[e] = S[i * ld + c];
+  }
+}
+
+static bool qr_small_fits(int m, int n) {
+  return m >= n && n >= 1 && n <= 256 && (int64_t)m * (n + 1) <= kQrSmallBig;
+}
+
+static int qr_small_b(double* A, int m, int n, int trans, double* R, int batch, hipStream_t s) {
+  TNX_CHECK_ARG(qr_small_fits(m, n), "qr_batched: %d x %d exceeds the LDS-resident limit", m, n);
+  TNX_CHECK_ARG(batch >= 1 && batch <= 2147483647, "qr_batched: bad batch %d", batch);
+  if ((int64_t)m * (n + 1) <= kQrSmallLittle)
+    hipLaunchKernelGGL(qr_small_kernel<kQrSmallLittle>, dim3(batch), dim3(256), 0, s, A, m, n,
+                       trans, R);
+  else
+    hipLaunchKernelGGL(qr_small_kernel<kQrSmallBig>, dim3(batch), dim3(256), 0, s, A, m, n, trans,
+                       R);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
+size_t gauge_b_ws_bytes(int chil, int d, int chir, int batch) {
+  const size_t n = (size_t)std::max(chil, chir);
+  return al(n * n * 8 * (size_t)batch) + 256;
+}
+
+int gauge_right_b(double* A, const double* next, double* next_out, int chil, int d, int chir,
+                  int next_cols, int batch, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(A && next && next_out && next != next_out, "gauge_right_batched: bad pointers");
+  TNX_CHECK_ARG(chil > 0 && d > 0 && chir > 0 && next_cols > 0 && batch > 0,
+                "gauge_right_batched: bad dims");
+  const int m = chil * d, n = chir;
+  TNX_CHECK_ARG(m >= n, "gauge_right_batched: chil*d=%d < chir=%d", m, n);
+  TNX_CHECK_ARG(ws && ws_bytes >= gauge_b_ws_bytes(chil, d, chir, batch),
+                "gauge_right_batched: workspace too small");
+  double* R = (double*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  TNX_TRY(qr_small_b(A, m, n, 0, R, batch, s));
+  GemmDesc g{};
+  g.M = n; g.N = next_cols; g.K = n; g.batch = batch;
+  g.A = R; g.ta = DT::F64; g.lda = n; g.sA = (int64_t)n * n; g.a_kcontig = true;
+  g.B = next; g.tb = DT::F64; g.ldb = next_cols; g.sB = (int64_t)n * next_cols; g.b_ncontig = true;
+  g.C = next_out; g.tc = DT::F64; g.ldc = next_cols; g.sC = (int64_t)n * next_cols;
+  g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+  return gemm(g, s);
+}
+
+int gauge_left_b(double* A, const double* prev, double* prev_out, int chil, int d, int chir,
+                 int prev_rows, int batch, void* ws, size_t ws_bytes, hipStream_t s) {
+  TNX_CHECK_ARG(A && prev && prev_out && prev != prev_out, "gauge_left_batched: bad pointers");
+  TNX_CHECK_ARG(chil > 0 && d > 0 && chir > 0 && prev_rows > 0 && batch > 0,
+                "gauge_left_batched: bad dims");
+  const int m = d * chir, n = chil;
+  TNX_CHECK_ARG(m >= n, "gauge_left_batched: d*chir=%d < chil=%d", m, n);
+  TNX_CHECK_ARG(ws && ws_bytes >= gauge_b_ws_bytes(chil, d, chir, batch),
+                "gauge_left_batched: workspace too small");
+  double* R = (double*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  TNX_TRY(qr_small_b(A, m, n, 1, R, batch, s));
+  // prev <- prev R^T
+  GemmDesc g{};
+  g.M = prev_rows; g.N = n; g.K = n; g.batch = batch;
+  g.A = prev; g.ta = DT::F64; g.lda = n; g.sA = (int64_t)prev_rows * n; g.a_kcontig = true;
+  g.B = R; g.tb = DT::F64; g.ldb = n; g.sB = (int64_t)n * n; g.b_ncontig = false;
+  g.C = prev_out; g.tc = DT::F64; g.ldc = n; g.sC = (int64_t)prev_rows * n;
+  g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+  return gemm(g, s);
+}
+
 }  // namespace tnx
 
 extern "C" {
@@ -420,6 +575,24 @@ int tnx_gauge_left(double* A, const double* prev, double* prev_out, int chil, in
                    int prev_rows, void* ws, size_t ws_bytes, tnx_stream_t stream) {
   return tnx::gauge_left(A, prev, prev_out, chil, d, chir, prev_rows, ws, ws_bytes,
                          (hipStream_t)stream);
+}
+
+size_t tnx_gauge_batched_workspace_bytes(int chil, int d, int chir, int batch) {
+  return tnx::gauge_b_ws_bytes(chil, d, chir, batch);
+}
+
+int tnx_gauge_right_batched(double* A, const double* next, double* next_out,
+                            int chil, int d, int chir, int next_cols, int batch,
+                            void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::gauge_right_b(A, next, next_out, chil, d, chir, next_cols, batch, ws, ws_bytes,
+                            (hipStream_t)stream);
+}
+
+int tnx_gauge_left_batched(double* A, const double* prev, double* prev_out,
+                           int chil, int d, int chir, int prev_rows, int batch,
+                           void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  return tnx::gauge_left_b(A, prev, prev_out, chil, d, chir, prev_rows, batch, ws, ws_bytes,
+                           (hipStream_t)stream);
 }
 
 }  // extern "C"

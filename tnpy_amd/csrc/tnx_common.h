@@ -78,6 +78,13 @@ struct GemmDesc {
   // has few tiles and K is long); null disables splitting.
   double* splitk_ws = nullptr;
   size_t splitk_ws_bytes = 0;
+  // Optional second batch level: batch index b = b1 * nb2 + b2 addresses
+  // A + b1*sA + b2*sA2 (same for B, C). nb2 = 1 means a single level.
+  int nb2 = 1;
+  int64_t sA2 = 0, sB2 = 0, sC2 = 0;
+  // Optional index list: launched batch i computes logical batch bidx[i]
+  // (a subset of a larger stack); null means launched == logical.
+  const int* bidx = nullptr;
 };
 
 int gemm(const GemmDesc& g, hipStream_t stream);
