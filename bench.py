@@ -59,7 +59,81 @@ def parse():
     p.add_argument("--keep", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=2)
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--batch-chains", type=int, default=256)
+    p.add_argument("--batch-chi", type=int, default=64)
+    p.add_argument("--batch-lengths", default="32,48,64,96")
+    p.add_argument("--batch-seed", type=int, default=7)
+    p.add_argument("--only-batched", action="store_true")
     return p.parse_args()
+
+
+def batched_leg(args, dev):
+    """Many-chain workload: random-coupling Heisenberg chains (J_i ~ U(0,1)
+    per bond from a per-chain seed), lengths drawn uniformly from
+    --batch-lengths, chi=--batch-chi, FP64, all swept together by
+    BatchedFiniteDMRG (one launch per kernel per site step and length
+    group). The same number of sweeps is then run chain by chain with
+    FiniteDMRG on one sample chain per length, from the same initial
+    states: their energies must agree, and their times scaled by the
+    number of chains of each length give the unbatched FP64 time."""
+    import torch
+
+    from tnpy_amd import Direction, FiniteDMRG
+    from tnpy_amd.batched_dmrg import BatchedFiniteDMRG, random_coupling_chains
+    from tnpy_amd.model import RandomCouplingHeisenberg
+
+    lengths = [int(x) for x in args.batch_lengths.split(",")]
+    ns, cseeds = random_coupling_chains(args.batch_chains, lengths, args.batch_seed)
+    models = [RandomCouplingHeisenberg(n=n, seed=sd) for n, sd in zip(ns, cseeds)]
+    chi = args.batch_chi
+    t = time.perf_counter()
+    bd = BatchedFiniteDMRG([m.mpo for m in models], chi=chi,
+                           seeds=[args.batch_seed * 100003 + i for i in range(len(ns))],
+                           max_basis=args.max_basis, keep=args.keep, device=dev)
+    torch.cuda.synchronize()
+    log(f"batched: {len(ns)} chains {dict(sorted((n, ns.count(n)) for n in set(ns)))} chi={chi} "
+        f"ready in {time.perf_counter() - t:.1f}s")
+    sample = [bd.groups[n][0] for n in sorted(bd.groups)]
+    starts = {i: bd.mps(i) for i in sample}
+    directions = [Direction.RIGHTWARD, Direction.LEFTWARD]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eb = []
+    for k in range(args.steps):
+        eb.append(bd.sweep(directions[k % 2], tol=args.tol))
+        torch.cuda.synchronize()
+        log(f"batched sweep {k + 1}: {time.perf_counter() - t0:.2f}s")
+    dt = time.perf_counter() - t0
+    single = {}
+    dEmax = 0.0
+    for i in sample:
+        f = FiniteDMRG(models[i].mpo, bond_dim=chi, mps=starts[i], max_basis=args.max_basis, keep=args.keep,
+                       compute_variance=False, device=dev)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        es = [f.sweep(directions[k % 2], tol=args.tol) for k in range(args.steps)]
+        torch.cuda.synchronize()
+        single[ns[i]] = time.perf_counter() - ts
+        dEmax = max(dEmax, abs(es[-1] - eb[-1][i]) / abs(es[-1]))
+    est = sum(single[n] for n in ns)
+    log(f"batched: {args.steps} sweep(s) of {len(ns)} chains in {dt:.2f}s; unbatched estimate {est:.1f}s "
+        f"(per-length single-chain sweep times {single}); max rel |dE| batched vs single {dEmax:.2e}")
+    return {
+        "workload": (f"{len(ns)} random-coupling Heisenberg chains, J_i~U(0,1), n in {lengths}, chi={chi}, "
+                     f"one-site DMRG sweep from seeded random states, FP64, tol={args.tol}"),
+        "chains": len(ns),
+        "lengths": {str(n): ns.count(n) for n in sorted(set(ns))},
+        "seconds": dt,
+        "sweeps_per_s": args.steps / dt,
+        "chain_sweeps_per_s": len(ns) * args.steps / dt,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "unbatched_fp64_est_seconds": est,
+        "speedup_vs_unbatched_fp64": est / dt,
+        "sample_rel_dE_vs_unbatched": dEmax,
+        "matvecs": bd.stats["matvecs"],
+        "lockstep_matvecs": bd.stats["lockstep_matvecs"],
+    }
 
 
 def sweep_timed(d, direction, label, step):
@@ -202,6 +276,9 @@ def main():
 
     logging.getLogger("tnpy").setLevel(logging.WARNING)
     dev = torch.device("cuda", 0)
+    if args.only_batched:
+        print(json.dumps({"batched": batched_leg(args, dev)}), flush=True)
+        return
     n, chi = args.n, args.chi
     mpo = XXZ(n=n, delta=args.delta).mpo
     log(f"workload XXZ n={n} delta={args.delta} chi={chi}, building seeded random MPS")
@@ -283,6 +360,10 @@ def main():
         }
         log(f"CPU baseline: {cpu['value']:.3e} sweeps/s ({c['sweep_s']:.1f} s/sweep est.)")
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args, dev)
+
     mixed = results[args.mixed]
     r = rates[args.mixed]
     K = args.steps
@@ -339,6 +420,7 @@ def main():
             "frac": rates["fp64"]["gemm_tflops"] / PEAK_TFLOPS["fp64"],
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

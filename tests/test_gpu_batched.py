@@ -351,3 +351,21 @@ def test_batched_dmrg_bulk_krylov(gpu):
     # run() keeps each chain's energies up to its own convergence
     hist = bd.run(tol=1e-9, max_sweep=4)
     assert all(1 <= len(h) <= 4 for h in hist)
+
+
+def test_batched_random_states_match_single(gpu):
+    """The batched initial states equal MatrixProductState.random with the
+    same seeds (same draws, LQ sweep and rescaling) to rounding."""
+    from tnpy_amd import MatrixProductState
+    from tnpy_amd.batched_dmrg import BatchedFiniteDMRG
+
+    ns = [10, 12, 10]
+    seeds = [5, 9, 13]
+    models = _chains(ns, 1)
+    bd = BatchedFiniteDMRG([m.mpo for m in models], bond_dim=16, seeds=seeds, device=gpu)
+    for i, n in enumerate(ns):
+        ref = MatrixProductState.random(n, 16, 2, seed=seeds[i], device=gpu, host_rng=True)
+        got = bd.mps(i)
+        for a, b in zip(got.tensors, ref.tensors):
+            assert a.shape == b.shape
+            assert (a - b).abs().max().item() <= 1e-12 * max(1.0, b.abs().max().item())

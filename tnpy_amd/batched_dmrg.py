@@ -29,14 +29,14 @@ import numpy as np
 import torch
 
 from tnpy_amd import _lib, logger
-from tnpy_amd.matrix_product_state import Direction, MatrixProductState
+from tnpy_amd.matrix_product_state import Direction, MatrixProductState, _bond_dims
 from tnpy_amd.operators import as_mpo
 
 
 class _Group:
     """Stacked state of the chains of one length."""
 
-    def __init__(self, owner: "BatchedFiniteDMRG", chains: List[int], mpos, starts):
+    def __init__(self, owner: "BatchedFiniteDMRG", chains: List[int], mpos, stacks: List[torch.Tensor]):
         self.owner = owner
         self.chains = chains
         self.B = len(chains)
@@ -46,7 +46,7 @@ class _Group:
             torch.stack([torch.as_tensor(m.padded[s], device=dev) for m in mpos]).contiguous()
             for s in range(self.n)
         ]
-        self.A = [torch.stack([st[s] for st in starts]).contiguous() for s in range(self.n)]
+        self.A = stacks
         one = torch.ones(self.B, 1, 1, 1, dtype=torch.float64, device=dev)
         self.left: Dict[int, torch.Tensor] = {0: one}
         self.right: Dict[int, torch.Tensor] = {self.n - 1: one.clone()}
@@ -272,27 +272,26 @@ class BatchedFiniteDMRG:
             raise ValueError("need one initial state per chain")
         if seeds is None:
             seeds = list(range(nchain))
-        starts = []
-        for i, m in enumerate(self._mpos):
-            if mps is None:
-                st = MatrixProductState.random(m.n_sites, self._bond_dim, d, seed=int(seeds[i]), device=self.device)
-            elif isinstance(mps[i], MatrixProductState):
-                st = mps[i]
-            else:
-                st = MatrixProductState(mps[i], device=self.device)
-            if st.n_sites != m.n_sites:
-                raise ValueError(f"chain {i}: MPO and MPS lengths differ")
-            starts.append(st.tensors)
         by_len: Dict[int, List[int]] = {}
         for i, m in enumerate(self._mpos):
             by_len.setdefault(m.n_sites, []).append(i)
         self._groups = []
         for n in sorted(by_len):
             idx = by_len[n]
-            shapes = {tuple(tuple(t.shape) for t in starts[i]) for i in idx}
-            if len(shapes) != 1:
-                raise ValueError(f"chains of length {n} have different bond dimensions")
-            self._groups.append(_Group(self, idx, [self._mpos[i] for i in idx], [starts[i] for i in idx]))
+            if mps is None:
+                stacks = _random_stacks(n, self._bond_dim, d, [int(seeds[i]) for i in idx], self)
+            else:
+                starts = []
+                for i in idx:
+                    st = mps[i] if isinstance(mps[i], MatrixProductState) else MatrixProductState(mps[i], device=self.device)
+                    if st.n_sites != n:
+                        raise ValueError(f"chain {i}: MPO and MPS lengths differ")
+                    starts.append(st.tensors)
+                shapes = {tuple(tuple(t.shape) for t in s) for s in starts}
+                if len(shapes) != 1:
+                    raise ValueError(f"chains of length {n} have different bond dimensions")
+                stacks = [torch.stack([s[k] for s in starts]).contiguous() for k in range(n)]
+            self._groups.append(_Group(self, idx, [self._mpos[i] for i in idx], stacks))
         self._energies: List[List[float]] = [[] for _ in range(nchain)]
         self.sweep_times: List[float] = []
 
@@ -360,6 +359,38 @@ class BatchedFiniteDMRG:
                 k = g.chains.index(chain)
                 return MatrixProductState([a[k].clone() for a in g.A], device=self.device)
         raise IndexError(chain)
+
+
+def _random_stacks(n: int, chi: int, d: int, seeds: Sequence[int], owner: "BatchedFiniteDMRG"):
+    """Stacked random initial states of equal-length chains: chain b's
+    entries are those of ``MatrixProductState.random(n, chi, d, seeds[b])``
+    (N(0,1) from ``numpy.random.default_rng(seed)``, site by site in C
+    order), then one batched LQ sweep from the right end with the same
+    per-site rescaling and final normalisation."""
+    dims = [1] + _bond_dims(n, chi, d) + [1]
+    shapes = [(dims[i], d, dims[i + 1]) for i in range(n)]
+    per = [[] for _ in range(n)]
+    for sd in seeds:
+        rng = np.random.default_rng(sd)
+        for k, sh in enumerate(shapes):
+            per[k].append(rng.standard_normal(sh))
+    A = [torch.as_tensor(np.stack(p), device=owner.device).contiguous() for p in per]
+    B = len(seeds)
+    lib = _lib.lib()
+    for site in range(n - 1, 0, -1):
+        _, l, p, r = A[site].shape
+        prv = A[site - 1]
+        out = torch.empty_like(prv)
+        wp, wb = owner._ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, B))
+        _lib.check(
+            lib.tnx_gauge_left_batched(
+                _lib.ptr(A[site], "site"), _lib.ptr(prv, "prev"), _lib.ptr(out, "prev_out"),
+                l, p, r, prv.shape[1] * prv.shape[2], B, wp, wb, owner._stream(),
+            ),
+            "tnx_gauge_left_batched",
+        )
+        A[site - 1] = (out / torch.linalg.vector_norm(out, dim=(1, 2, 3), keepdim=True)).contiguous()
+    return A
 
 
 def random_coupling_chains(n_chains: int, lengths: Sequence[int], seed: int):
