@@ -65,7 +65,22 @@ def parse():
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--only-batched", action="store_true")
+    p.add_argument("--kernels-only", action="store_true",
+                   help="only the HIP-event kernel rates at chi, plus a plain 256 MiB device copy "
+                        "(the reference kernel for scaling HBM counters)")
     return p.parse_args()
+
+
+def copy_reference(mib=256, reps=10):
+    """A plain device-to-device copy of known size (torch's copy kernel):
+    reads and writes mib MiB per launch."""
+    import torch
+
+    x = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    ms = event_time(lambda: y.copy_(x), reps)
+    return {"bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8, "ms": ms,
+            "GBps": 2 * x.numel() * 8 / ms / 1e6}
 
 
 def batched_leg(args, dev):
@@ -217,6 +232,21 @@ def kernel_rates(chi, D, d, precisions):
     return out
 
 
+def measured_traffic(precision, M, N, K):
+    """HBM bytes per launch of the stage-1 GEMM from the committed counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by a plain
+    copy's reading), or None when no summary matches this shape."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_stage1_chi1024_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if (t.get("precision"), t.get("M"), t.get("N"), t.get("K")) != (precision, M, N, K):
+        return None
+    return t["bytes_per_launch"]
+
+
 def cpu_baseline(chi, D, d, n, mv_per_site, bond, reps):
     """Bounded CPU sample: the oracle's matvec, SVD gauge move and
     environment update at chi x chi, each timed ``reps`` times on the host
@@ -278,6 +308,10 @@ def main():
     dev = torch.device("cuda", 0)
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(args, dev)}), flush=True)
+        return
+    if args.kernels_only:
+        rates = kernel_rates(args.chi, 5, 2, ["fp64", args.mixed])
+        print(json.dumps({"chi": args.chi, "rates": rates, "copy": copy_reference()}), flush=True)
         return
     n, chi = args.n, args.chi
     mpo = XXZ(n=n, delta=args.delta).mpo
@@ -418,6 +452,7 @@ def main():
             "peak": PEAK_TFLOPS["fp64"],
             "unit": "TFLOP/s",
             "frac": rates["fp64"]["gemm_tflops"] / PEAK_TFLOPS["fp64"],
+            "traffic": measured_traffic("fp64", 2 * chi, 5 * chi, chi),
         },
         "cpu_baseline": cpu,
         "batched": batched,
