@@ -112,7 +112,8 @@ static size_t inter_bytes(int prec) { return prec == TNX_F64 ? 8 : 4; }
 // Workspace: T1, T2, and FP64 split-K partials for the last GEMM (its
 // output, chil x d*chir, has few tiles at chil = chir and K = chil*Dl is
 // long; the split rule in gemm.hip decides how many are used).
-static size_t splitk_bytes(int64_t out_elems) { return (size_t)out_elems * 2 * 8; }
+// partial sums of up to 4 K-splits of an output of out_elems doubles
+static size_t splitk_bytes(int64_t out_elems) { return (size_t)out_elems * 4 * 8; }
 
 size_t heff1_ws_bytes_b(int chil, int Dl, int Dr, int d, int chir, int batch) {
   const size_t t1 = (size_t)chil * d * Dr * chir;

@@ -9,14 +9,13 @@
 //            a*b ~ ahi*bhi + ahi*blo + alo*bhi with a = ahi + alo (bf16),
 //            FP32 accumulate, LDS tiles [BM][BK] so each lane reads its
 //            8 consecutive k values with one 16-byte LDS read.
-// K-tile depth: 32 for FP64 (one 16x16x4 FP64 MFMA is 64 cycles, so a
-// deeper tile hides the HBM latency of the next tile's loads at one wave per
-// SIMD), 16 for FP32, 32 for BF16X3.
-// Operands may be FP64 or FP32 in HBM (runtime, wave-uniform branch); they
-// are converted while staging the tile in LDS. Block = 256 threads as a 2x2
-// grid of 64-wide wavefronts, each wave owning a (BM/2)x(BN/2) sub-tile of
-// 16x16 MFMA accumulators. Global loads of tile t+1 are issued before the
-// MFMAs of tile t; LDS is double-buffered (one barrier per k-tile).
+// Tile configurations (block tile BM x BN, WGM x WGN wavefronts of 64 lanes,
+// K-tile depth BK) are listed in Cfgs below; the autotuner picks one per
+// shape. Each wave owns a (BM/WGM) x (BN/WGN) sub-tile of 16x16 MFMA
+// accumulators. Operands may be FP64 or FP32 in HBM (runtime, wave-uniform
+// branch); they are converted while staging the tile in LDS. Global loads of
+// tile t+1 are issued before the MFMAs of tile t and their results are first
+// used after them; LDS is double-buffered (one barrier per k-tile).
 #include "tnx_common.h"
 
 #include <cmath>
@@ -24,12 +23,6 @@
 #include <mutex>
 #include <unordered_map>
 
-#ifndef TNX_BK64
-#define TNX_BK64 32
-#endif
-#ifndef TNX_BK32
-#define TNX_BK32 16
-#endif
 
 namespace tnx {
 
@@ -51,15 +44,10 @@ struct GemmArgs {
   const int* bidx;  // launched -> logical batch index, or null
 };
 
-__device__ __forceinline__ double ld_any(const void* p, int t, int64_t i) {
-  return t == 0 ? reinterpret_cast<const double*>(p)[i]
-                : (double)reinterpret_cast<const float*>(p)[i];
-}
-
 template <int PREC> struct PT;
-template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; using accs_t = double; static constexpr int BK = TNX_BK64; };
-template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; using accs_t = float;  static constexpr int BK = TNX_BK32; };
-template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; using accs_t = float;  static constexpr int BK = 32; };
+template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; };
+template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; };
+template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; };
 
 // WGM x WGN wavefronts per block; each wave owns a (BM/WGM) x (BN/WGN) tile.
 template <int PREC, int BM, int BN, bool A_K, bool B_N, int WGM, int WGN, int BK>
@@ -101,8 +89,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   const char* Ab = (const char*)g.A + ((size_t)b1 * g.sA + (size_t)b2 * g.sA2) * (g.ta == 0 ? 8 : 4);
   const char* Bb = (const char*)g.B + ((size_t)b1 * g.sB + (size_t)b2 * g.sB2) * (g.tb == 0 ? 8 : 4);
 
-  float ra[EA], rb[EB];
-  double da[PREC == 0 ? EA : 1], db[PREC == 0 ? EB : 1];
+  // Staging registers hold the raw loaded bits (a double, or a float in the
+  // low word) so that no instruction uses a global load's result until
+  // store_tiles, after the MFMAs of the current k-block: the loads of
+  // block k+1 stay in flight under the compute of block k.
+  uint64_t ua[EA], ub[EB];
 
   auto load_tiles = [&](int k0) {
 #pragma unroll
@@ -111,12 +102,13 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
       int mi, ki;
       if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
       const int m = m0 + mi, k = k0 + ki;
-      double v = 0.0;
+      uint64_t v = 0;
       if (m < g.M && k < kend) {
         const int64_t off = A_K ? (int64_t)m * g.lda + k : (int64_t)k * g.lda + m;
-        v = ld_any(Ab, g.ta, off);
+        if (g.ta == 0) v = reinterpret_cast<const uint64_t*>(Ab)[off];
+        else v = reinterpret_cast<const uint32_t*>(Ab)[off];
       }
-      if constexpr (PREC == 0) da[e] = v; else ra[e] = (float)v;
+      ua[e] = v;
     }
 #pragma unroll
     for (int e = 0; e < EB; ++e) {
@@ -124,13 +116,20 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
       int ni, ki;
       if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
       const int n = n0 + ni, k = k0 + ki;
-      double v = 0.0;
+      uint64_t v = 0;
       if (n < g.N && k < kend) {
         const int64_t off = B_N ? (int64_t)k * g.ldb + n : (int64_t)n * g.ldb + k;
-        v = ld_any(Bb, g.tb, off);
+        if (g.tb == 0) v = reinterpret_cast<const uint64_t*>(Bb)[off];
+        else v = reinterpret_cast<const uint32_t*>(Bb)[off];
       }
-      if constexpr (PREC == 0) db[e] = v; else rb[e] = (float)v;
+      ub[e] = v;
     }
+  };
+  auto as_double = [](uint64_t v, int t) -> double {
+    return t == 0 ? __longlong_as_double((long long)v) : (double)__uint_as_float((uint32_t)v);
+  };
+  auto as_float = [](uint64_t v, int t) -> float {
+    return t == 0 ? (float)__longlong_as_double((long long)v) : __uint_as_float((uint32_t)v);
   };
 
   auto store_tiles = [&](int buf) {
@@ -140,11 +139,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
       int mi, ki;
       if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
       if constexpr (PREC == 0) {
-        As[buf][ki * (BM + PADM) + mi] = da[e];
+        As[buf][ki * (BM + PADM) + mi] = as_double(ua[e], g.ta);
       } else if constexpr (PREC == 1) {
-        As[buf][ki * (BM + PADM) + mi] = ra[e];
+        As[buf][ki * (BM + PADM) + mi] = as_float(ua[e], g.ta);
       } else {
-        const float x = ra[e];
+        const float x = as_float(ua[e], g.ta);
         const __bf16 hi = (__bf16)x;
         const __bf16 lo = (__bf16)(x - (float)hi);
         As[buf][mi * (BK + PADK) + ki] = hi;
@@ -157,11 +156,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
       int ni, ki;
       if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
       if constexpr (PREC == 0) {
-        Bs[buf][ki * (BN + PADM) + ni] = db[e];
+        Bs[buf][ki * (BN + PADM) + ni] = as_double(ub[e], g.tb);
       } else if constexpr (PREC == 1) {
-        Bs[buf][ki * (BN + PADM) + ni] = rb[e];
+        Bs[buf][ki * (BN + PADM) + ni] = as_float(ub[e], g.tb);
       } else {
-        const float x = rb[e];
+        const float x = as_float(ub[e], g.tb);
         const __bf16 hi = (__bf16)x;
         const __bf16 lo = (__bf16)(x - (float)hi);
         Bs[buf][ni * (BK + PADK) + ki] = hi;
@@ -323,8 +322,8 @@ template <> struct Cfgs<0> {
 };
 template <> struct Cfgs<1> {
   static constexpr int n = 6;
-  static constexpr Cfg c[n] = {{64, 64, 2, 2, 16}, {128, 128, 2, 2, 16}, {256, 128, 4, 2, 16},
-                               {128, 256, 2, 4, 16}, {256, 160, 4, 2, 16}, {160, 256, 2, 4, 16}};
+  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 32},
+                               {128, 256, 2, 4, 32}, {256, 160, 4, 2, 32}, {160, 256, 2, 4, 32}};
 };
 template <> struct Cfgs<2> {
   static constexpr int n = 6;
@@ -365,11 +364,12 @@ static void launch_index(int ci, const GemmDesc& d, const GemmArgs& g, int split
 }
 
 // Deterministic split-K rule: split only when the output is small (fewer
-// than ~256 tiles of 128x128) and K is long, and the scratch fits.
+// than ~512 tiles of 128x128) and K is long, and the scratch fits.
 static int split_rule(const GemmDesc& d) {
   if (!d.splitk_ws) return 1;
   const int64_t elems = (int64_t)d.M * d.N * d.batch;
-  const int64_t target = (int64_t)256 * 128 * 128;
+  // about two 256x128 output tiles per CU after splitting
+  const int64_t target = (int64_t)2 * 256 * 128 * 128;
   if (elems >= target || d.K < 1024) return 1;
   int sp = 1;
   while (sp < 32 && elems * sp * 2 <= target && (int64_t)d.K / (sp * 2) >= 256) sp *= 2;
@@ -457,13 +457,16 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
     if (autotune_on() && work >= ((int64_t)1 << 28) && !d.bidx) {
       // Time every config once on a scratch output (the caller's C is not
       // touched), keep the fastest. Runs once per shape key.
+      // Split-K launches only write the partial-sum scratch (which the real
+      // launch below overwrites), so they are timed in place; others write a
+      // scratch output.
       GemmArgs gt = g;
       void* scratch = nullptr;
       const int nb2 = g.nb2;
       const size_t cbytes = ((size_t)((d.batch - 1) / nb2) * d.sC + (size_t)((d.batch - 1) % nb2) * d.sC2 +
                              (size_t)(d.M - 1) * d.ldc + d.N) * (d.tc == DT::F64 ? 8 : 4);
-      if (!g.partial && hipMalloc(&scratch, cbytes) == hipSuccess) {
-        gt.C = scratch;
+      if (g.partial || hipMalloc(&scratch, cbytes) == hipSuccess) {
+        if (scratch) gt.C = scratch;
         gt.beta = 0.0;
         hipEvent_t e0, e1;
         hipEventCreate(&e0);
@@ -483,7 +486,7 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         hipStreamSynchronize(s);
-        hipFree(scratch);
+        if (scratch) hipFree(scratch);
         hipGetLastError();
       }
     }

@@ -250,15 +250,30 @@ __global__ void copy_map_kernel(const double* __restrict__ in, double* __restric
 // ---------------------------------------------------------------------------
 constexpr int kTriMax = 256;
 
-__device__ int sturm_count(const double* a, const double* b2, int n, double x, double pivmin) {
+// Number of eigenvalues of the tridiagonal (a, b) below x: sign changes
+// of the leading principal minors p_k = det(T_k - x I) (p_0 = 1), by the
+// three-term recurrence p_{k+1} = (a_k - x) p_k - b_{k-1}^2 p_{k-1}. It has
+// no division (two dependent FMAs per step, against a division per step for
+// the LDL^T pivots); every 4 steps both carried minors are rescaled by a
+// power of two so they neither overflow nor underflow. An exact zero minor
+// takes the sign opposite to its predecessor (the pivot convention q = -0).
+__device__ int sturm_count(const double* a, const double* b2, int n, double x) {
+  double pm = 1.0, p = a[0] - x;
+  bool neg_prev = false;
   int cnt = 0;
-  double q = a[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  cnt += (q < 0.0);
-  for (int i = 1; i < n; ++i) {
-    q = a[i] - x - b2[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += (q < 0.0);
+  for (int i = 0;; ++i) {
+    const bool neg = (p < 0.0) || (p == 0.0 && !neg_prev);
+    cnt += (neg != neg_prev);
+    neg_prev = neg;
+    if (i + 1 >= n) break;
+    const double pn = fma(a[i + 1] - x, p, -b2[i] * pm);
+    pm = p;
+    p = pn;
+    if ((i & 3) == 3) {
+      const int e = __builtin_amdgcn_frexp_exp(p == 0.0 ? pm : p);
+      p = ldexp(p, -e);
+      pm = ldexp(pm, -e);
+    }
   }
   return cnt;
 }
@@ -275,6 +290,8 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
   __shared__ double u0[kTriMax], u1[kTriMax], u2[kTriMax], z[kTriMax];
+  __shared__ double fm[kTriMax], rinv[kTriMax];
+  __shared__ int swp[kTriMax];
   __shared__ double shr[4];
   const int lane = threadIdx.x;
   for (int i = lane; i < n; i += 64) {
@@ -293,27 +310,26 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
     }
     return;
   }
-  // Gershgorin bounds.
-  if (lane == 0) {
-    double lo = 1e308, amin = 1e308, tn = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0);
-      lo = fmin(lo, a[i] - r);
-      amin = fmin(amin, a[i]);
-      tn = fmax(tn, fabs(a[i]) + r);
-    }
-    shr[0] = lo; shr[1] = amin; shr[2] = tn;
+  // Gershgorin bounds (lanes over rows, then wave min/max).
+  double lo = 1e308, hi = 1e308, tnorm = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < n - 1 ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmin(hi, a[i]);
+    tnorm = fmax(tnorm, fabs(a[i]) + r);
   }
-  __syncthreads();
-  double lo = shr[0], hi = shr[1];
-  const double tnorm = shr[2];
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmin(hi, __shfl_xor(hi, o, 64));
+    tnorm = fmax(tnorm, __shfl_xor(tnorm, o, 64));
+  }
   const double pivmin = fmax(tnorm, 1e-300) * 1e-300 + 1e-300;
   const double eps = 2.220446049250313e-16;
   // 64-way multisection: each lane evaluates the Sturm count at one point.
   for (int it = 0; it < 12; ++it) {
     if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-    const int c = sturm_count(a, b2, n, x, pivmin);
+    const int c = sturm_count(a, b2, n, x);
     const unsigned long long mask = __ballot(c >= 1);
     if (mask == 0ull) {
       lo = __shfl(x, 63, 64);
@@ -326,49 +342,67 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
     }
   }
   const double theta = 0.5 * (lo + hi);
-  // Inverse iteration on (T - theta I), Gaussian elimination with partial
-  // pivoting (lane 0), zero pivots replaced by eps*||T||.
+  // Inverse iteration on (T - theta I): Gaussian elimination with partial
+  // pivoting (lane 0; zero pivots replaced by eps*||T||) factors once,
+  // keeping multipliers and row swaps; the two refinement passes then only
+  // apply them to the right-hand side and back-substitute with reciprocal
+  // pivots computed by all lanes.
+  const double tiny = eps * fmax(tnorm, 1e-300);
+  for (int i = lane; i < n; i += 64) z[i] = 1.0 + 0.01 * (double)((i * 7919) % 13);
   if (lane == 0) {
-    const double tiny = eps * fmax(tnorm, 1e-300);
-    for (int i = 0; i < n; ++i) z[i] = 1.0 + 0.01 * (double)((i * 7919) % 13);
-    for (int rep = 0; rep < 2; ++rep) {
-      // current pivot row: (c0, c1, c2) at columns i, i+1, i+2
-      double c0 = a[0] - theta, c1 = b[0], c2 = 0.0, rc = z[0];
-      for (int i = 0; i < n - 1; ++i) {
-        const double d0 = b[i], d1 = a[i + 1] - theta, d2 = (i + 1 < n - 1) ? b[i + 1] : 0.0;
-        double rd = z[i + 1];
-        if (fabs(c0) >= fabs(d0)) {
-          if (c0 == 0.0) c0 = tiny;
-          const double f = d0 / c0;
-          u0[i] = c0; u1[i] = c1; u2[i] = c2; z[i] = rc;
-          c0 = d1 - f * c1; c1 = d2 - f * c2; c2 = 0.0; rc = rd - f * rc;
-        } else {
-          const double f = c0 / d0;
-          u0[i] = d0; u1[i] = d1; u2[i] = d2; z[i] = rd;
-          const double n0 = c1 - f * d1, n1 = c2 - f * d2;
-          c0 = n0; c1 = n1; c2 = 0.0; rc = rc - f * rd;
-        }
+    // current pivot row: (c0, c1, c2) at columns i, i+1, i+2
+    double c0 = a[0] - theta, c1 = b[0], c2 = 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+      const double d0 = b[i], d1 = a[i + 1] - theta, d2 = (i + 1 < n - 1) ? b[i + 1] : 0.0;
+      if (fabs(c0) >= fabs(d0)) {
+        if (c0 == 0.0) c0 = tiny;
+        const double f = d0 / c0;
+        u0[i] = c0; u1[i] = c1; u2[i] = c2; fm[i] = f; swp[i] = 0;
+        c0 = d1 - f * c1; c1 = d2 - f * c2; c2 = 0.0;
+      } else {
+        const double f = c0 / d0;
+        u0[i] = d0; u1[i] = d1; u2[i] = d2; fm[i] = f; swp[i] = 1;
+        const double n0 = c1 - f * d1, n1 = c2 - f * d2;
+        c0 = n0; c1 = n1; c2 = 0.0;
       }
-      if (c0 == 0.0) c0 = tiny;
-      u0[n - 1] = c0; u1[n - 1] = 0.0; u2[n - 1] = 0.0; z[n - 1] = rc;
-      // back substitution
-      z[n - 1] = z[n - 1] / u0[n - 1];
-      if (n >= 2) z[n - 2] = (z[n - 2] - u1[n - 2] * z[n - 1]) / u0[n - 2];
-      for (int i = n - 3; i >= 0; --i) z[i] = (z[i] - u1[i] * z[i + 1] - u2[i] * z[i + 2]) / u0[i];
-      double nn = 0.0;
-      for (int i = 0; i < n; ++i) nn += z[i] * z[i];
-      const double inv = 1.0 / sqrt(nn);
-      for (int i = 0; i < n; ++i) z[i] *= inv;
     }
+    if (c0 == 0.0) c0 = tiny;
+    u0[n - 1] = c0; u1[n - 1] = 0.0; u2[n - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) rinv[i] = 1.0 / u0[i];
+  __syncthreads();
+  for (int rep = 0; rep < 2; ++rep) {
+    if (lane == 0) {
+      double rc = z[0];
+      for (int i = 0; i < n - 1; ++i) {
+        const double rd = z[i + 1];
+        if (swp[i]) { z[i] = rd; rc = rc - fm[i] * rd; }
+        else { z[i] = rc; rc = rd - fm[i] * rc; }
+      }
+      z[n - 1] = rc * rinv[n - 1];
+      if (n >= 2) z[n - 2] = (z[n - 2] - u1[n - 2] * z[n - 1]) * rinv[n - 2];
+      for (int i = n - 3; i >= 0; --i) z[i] = (z[i] - u1[i] * z[i + 1] - u2[i] * z[i + 2]) * rinv[i];
+    }
+    __syncthreads();
+    double nn = 0.0;
+    for (int i = lane; i < n; i += 64) nn += z[i] * z[i];
+    const double inv = 1.0 / sqrt(wave_sum(nn));
+    for (int i = lane; i < n; i += 64) z[i] *= inv;
+    __syncthreads();
+  }
+  if (lane == 0) {
     // fix the sign: largest component positive (deterministic output)
     int im = 0;
     for (int i = 1; i < n; ++i) if (fabs(z[i]) > fabs(z[im])) im = i;
-    const double sg = z[im] < 0.0 ? -1.0 : 1.0;
-    for (int i = 0; i < n; ++i) s[i] = sg * z[i];
+    shr[0] = z[im] < 0.0 ? -1.0 : 1.0;
     st[0] = theta;
     st[1] = bres ? fabs(*bres * z[n - 1]) : 0.0;
     st[2] = tnorm;
   }
+  __syncthreads();
+  const double sg = shr[0];
+  for (int i = lane; i < n; i += 64) s[i] = sg * z[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -968,6 +1002,11 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
     have_hy = true;
     anorm = std::max(anorm, std::fabs(theta));
     if (cprec != TNX_F64 && rho_prev > 0.0 && rho > 0.25 * rho_prev) cprec = TNX_F64;
+    // After the first low-precision cycle (run to its arithmetic's floor), a
+    // tolerance below that floor is reached in FP64: a further low-precision
+    // cycle would restart the Krylov space for a target it cannot resolve.
+    if (cprec != TNX_F64 && !first_cycle && tol < 8.0 * (cprec == TNX_F32 ? 5.96e-8 : 3.9e-6))
+      cprec = TNX_F64;
     rho_prev = rho;
     const bool mixed = cprec != TNX_F64;
     if (rho <= tol * anorm || rho == 0.0 || N == 1) {
