@@ -128,10 +128,11 @@ size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
 // y_p = H_eff(L_p, W_p, R_p) x_p for the problems p of a stack of `batch`:
 // all of them (bidx null) or the `count` listed in bidx. L, W, R are
 // contiguous stacks; x_p = x + p*sx, y_p = y + p*sy (sx = sy = chil*d*chir
-// for contiguous stacks). One problem: batch = count = 1.
-int heff1_matvec_b(const double* L, const double* W, const double* R, const double* x, double* y,
-                   int chil, int Dl, int Dr, int d, int chir, int prec, int batch, int count,
-                   const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
+// for contiguous stacks). One problem: batch = count = 1. x is FP64, or
+// FP32 when x_f32 (a low-precision Krylov basis); y is FP64.
+int heff1_matvec_b(const double* L, const double* W, const double* R, const void* x, int x_f32,
+                   double* y, int chil, int Dl, int Dr, int d, int chir, int prec, int batch,
+                   int count, const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
                    hipStream_t s) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "heff1_matvec: bad dims");
   TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "heff1_matvec: bad batch %d", batch);
@@ -151,7 +152,7 @@ int heff1_matvec_b(const double* L, const double* W, const double* R, const doub
   // T1[(a,k),(v,s)] = X[(a,k), r] R[r, (v,s)]
   GemmDesc g1{};
   g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = count; g1.bidx = bidx;
-  g1.A = x; g1.ta = DT::F64; g1.lda = chir; g1.sA = sx; g1.a_kcontig = true;
+  g1.A = x; g1.ta = x_f32 ? DT::F32 : DT::F64; g1.lda = chir; g1.sA = sx; g1.a_kcontig = true;
   g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = (int64_t)chir * Dr * chir; g1.b_ncontig = true;
   g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = nt1;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
@@ -182,7 +183,7 @@ int heff1_matvec(const double* L, const double* W, const double* R,
                  const double* x, double* y, int chil, int Dl, int Dr, int d,
                  int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s) {
   const int64_t nx = (int64_t)chil * d * chir;
-  return heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, prec, 1, 1, nullptr, nx, nx, ws,
+  return heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, prec, 1, 1, nullptr, nx, nx, ws,
                         ws_bytes, s);
 }
 
@@ -202,8 +203,8 @@ size_t heff2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir) {
 }
 
 int heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
-                 const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d, int chir,
-                 int prec, void* ws, size_t ws_bytes, hipStream_t s) {
+                 const void* x, int x_f32, double* y, int chil, int Dl, int Dm, int Dr, int d,
+                 int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s) {
   TNX_CHECK_ARG(chil > 0 && Dl > 0 && Dm > 0 && Dr > 0 && d > 0 && chir > 0, "heff2_matvec: bad dims");
   TNX_CHECK_ARG(L && W1 && W2 && R && x && y, "heff2_matvec: null pointer");
   TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "heff2_matvec: bad precision");
@@ -218,7 +219,7 @@ int heff2_matvec(const double* L, const double* W1, const double* W2, const doub
   char* T3 = T1;
   GemmDesc g1{};
   g1.M = chil * d * d; g1.N = Dr * chir; g1.K = chir; g1.batch = 1;
-  g1.A = x; g1.ta = DT::F64; g1.lda = chir; g1.sA = 0; g1.a_kcontig = true;
+  g1.A = x; g1.ta = x_f32 ? DT::F32 : DT::F64; g1.lda = chir; g1.sA = 0; g1.a_kcontig = true;
   g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = 0; g1.b_ncontig = true;
   g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = 0;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
@@ -487,7 +488,7 @@ size_t tnx_heff2_workspace_bytes(int chil, int Dl, int Dm, int Dr, int d, int ch
 int tnx_heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
                      const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d,
                      int chir, int precision, void* ws, size_t ws_bytes, tnx_stream_t stream) {
-  return tnx::heff2_matvec(L, W1, W2, R, x, y, chil, Dl, Dm, Dr, d, chir, precision, ws,
+  return tnx::heff2_matvec(L, W1, W2, R, x, 0, y, chil, Dl, Dm, Dr, d, chir, precision, ws,
                            ws_bytes, (hipStream_t)stream);
 }
 
@@ -514,7 +515,7 @@ int tnx_heff1_matvec_batched(const double* L, const double* W, const double* R,
                              int d, int chir, int batch, int precision, void* ws,
                              size_t ws_bytes, tnx_stream_t stream) {
   const int64_t nx = (int64_t)chil * d * chir;
-  return tnx::heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, precision, batch, batch,
+  return tnx::heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, precision, batch, batch,
                              nullptr, nx, nx, ws, ws_bytes, (hipStream_t)stream);
 }
 

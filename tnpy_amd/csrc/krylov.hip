@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace tnx {
@@ -32,8 +33,12 @@ int heff1_matvec(const double* L, const double* W, const double* R, const double
                  void* ws, size_t ws_bytes, hipStream_t s);
 size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir);
 int heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
-                 const double* x, double* y, int chil, int Dl, int Dm, int Dr, int d, int chir,
-                 int prec, void* ws, size_t ws_bytes, hipStream_t s);
+                 const void* x, int x_f32, double* y, int chil, int Dl, int Dm, int Dr, int d,
+                 int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s);
+int heff1_matvec_b(const double* L, const double* W, const double* R, const void* x, int x_f32,
+                   double* y, int chil, int Dl, int Dr, int d, int chir, int prec, int batch,
+                   int count, const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
+                   hipStream_t s);
 size_t heff2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir);
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -61,8 +66,8 @@ constexpr int kCgsBlocks = 512;      // blocks of a CGS pass (partials rows)
 //   if hsub:  w -= sum_j V[j] * hsub[j]               (in place)
 //   partial[blk][j] = sum_i V[j][i] w[i]   (j < J, if DOTS)
 //   partial[blk][JMAX... stored at index J] = sum_i w[i]^2
-template <int JMAX, bool DOTS>
-__global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict__ V,
+template <int JMAX, bool DOTS, typename VT>
+__global__ __launch_bounds__(256) void cgs_pass_kernel(const VT* __restrict__ V,
                                                        int64_t ldv, int J,
                                                        double* __restrict__ w,
                                                        const double* __restrict__ hsub,
@@ -84,8 +89,20 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const double* __restrict_
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
     double vv[JMAX];
+    if constexpr (std::is_same<VT, double>::value) {
 #pragma unroll
-    for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? V[(int64_t)j * ldv + i] : 0.0;
+      for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? V[(int64_t)j * ldv + i] : 0.0;
+    } else {
+      // FP32 basis: unconditional loads (rows past J re-read row J-1, a cache
+      // hit) and the widening after all of them. A guarded load with its
+      // conversion in the same branch waits for its data before the next
+      // load issues.
+      VT raw[JMAX];
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) raw[j] = V[(int64_t)(j < J ? j : J - 1) * ldv + i];
+#pragma unroll
+      for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? (double)raw[j] : 0.0;
+    }
     double wi = w[i];
     if (hsub) {
 #pragma unroll
@@ -135,7 +152,10 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* part
   if (threadIdx.x == 0) out[j] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double* hsub,
+// One CGS pass over a basis stored in VT (double, or float for the basis of
+// a low-precision Krylov cycle); w and all sums are FP64.
+template <typename VT>
+static int cgs_pass(const VT* V, int64_t ldv, int J, double* w, const double* hsub,
                     bool dots, double* partials, double* out, int64_t N, hipStream_t s,
                     PMap pm = kOne, int count = 1) {
   const int nblk = (int)std::min<int64_t>(count > 1 ? std::max(kCgsBlocks / count, 8) : kCgsBlocks,
@@ -143,8 +163,8 @@ static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double
   auto go = [&](auto jtag, auto dtag) {
     constexpr int JM = decltype(jtag)::value;
     constexpr bool DT_ = decltype(dtag)::value;
-    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_>), dim3(nblk, count), dim3(256), 0, s, V, ldv, J,
-                       w, hsub, partials, N, pm);
+    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_, VT>), dim3(nblk, count), dim3(256), 0, s, V, ldv,
+                       J, w, hsub, partials, N, pm);
   };
   using T8 = std::integral_constant<int, 8>;
   using T16 = std::integral_constant<int, 16>;
@@ -165,7 +185,8 @@ static int cgs_pass(const double* V, int64_t ldv, int J, double* w, const double
 }
 
 // out[i] = sum_j c[j] V[j][i]   (Ritz vector from the basis)
-__global__ void vcomb_kernel(const double* __restrict__ V, int64_t ldv, int J,
+template <typename VT>
+__global__ void vcomb_kernel(const VT* __restrict__ V, int64_t ldv, int J,
                              const double* __restrict__ c, double* __restrict__ out, int64_t N,
                              PMap pm) {
   {
@@ -178,14 +199,15 @@ __global__ void vcomb_kernel(const double* __restrict__ V, int64_t ldv, int J,
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int j = 0; j < J; ++j) acc += cs[j] * V[(int64_t)j * ldv + i];
+    for (int j = 0; j < J; ++j) acc += cs[j] * (double)V[(int64_t)j * ldv + i];
     out[i] = acc;
   }
 }
 
 // out = x / sqrt(*nrm2)  (unit vector; zero if the norm underflows)
+template <typename OT>
 __global__ void normalize_kernel(const double* __restrict__ x, const double* __restrict__ nrm2,
-                                 double* __restrict__ out, int64_t N, PMap pm) {
+                                 OT* __restrict__ out, int64_t N, PMap pm) {
   {
     const int64_t o = pm.off();
     x += o; nrm2 += o; out += o;
@@ -194,7 +216,14 @@ __global__ void normalize_kernel(const double* __restrict__ x, const double* __r
   const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = x[i] * inv;
+    out[i] = (OT)(x[i] * inv);
+}
+
+// out = (float)x (the first vector of a low-precision Krylov basis)
+__global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ out, int64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)x[i];
 }
 
 // out = (*a) * x + (*b) * z, scalars read from device memory. Batched:
@@ -555,10 +584,11 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
 
 // ---------------------------------------------------------------------------
 // All eigenpairs of a symmetric tridiagonal matrix (n <= 64): implicit QL
-// with Wilkinson shifts, one wavefront. Every lane runs the scalar
-// recurrence on the LDS copies of the diagonal / off-diagonal (identical
-// values); lane k owns row k of the eigenvector matrix Z and applies each
-// Givens rotation to it. Eigenvalues ascending to w, eigenvectors as columns
+// with Wilkinson shifts, one wavefront. Every lane runs the same scalar
+// recurrence on the LDS copies of the diagonal / off-diagonal and writes the
+// same values to them, so each lane only ever reads its own writes and the
+// recurrence needs no barrier; lane k owns row k of the eigenvector matrix Z
+// and applies each Givens rotation to it. Eigenvalues ascending to w, eigenvectors as columns
 // of S (row stride ldS). Used at thick restarts, where the projected matrix
 // is tridiagonal.
 // ---------------------------------------------------------------------------
@@ -598,25 +628,22 @@ __global__ __launch_bounds__(64) void tridiag_all_kernel(const double* __restric
       for (; i >= l; --i) {
         double f = sn * e[i];
         const double b = cs * e[i];
-        r = hypot(f, g);
-        __syncthreads();
-        if (k == 0) e[i + 1] = r;
-        __syncthreads();
+        // |f|, |g| <= ||T||: no scaling needed for the rotation length
+        r = sqrt(fma(f, f, g * g));
+        e[i + 1] = r;
         if (r == 0.0) {
-          __syncthreads();
-          if (k == 0) { d[i + 1] -= p; e[m] = 0.0; }
-          __syncthreads();
+          d[i + 1] -= p;
+          e[m] = 0.0;
           early = true;
           break;
         }
-        sn = f / r;
-        cs = g / r;
+        const double ir = 1.0 / r;
+        sn = f * ir;
+        cs = g * ir;
         g = d[i + 1] - p;
         r = (d[i] - g) * sn + 2.0 * cs * b;
         p = sn * r;
-        __syncthreads();
-        if (k == 0) d[i + 1] = g + p;
-        __syncthreads();
+        d[i + 1] = g + p;
         g = cs * r - b;
         if (k < n) {
           f = Z[k][i + 1];
@@ -625,9 +652,9 @@ __global__ __launch_bounds__(64) void tridiag_all_kernel(const double* __restric
         }
       }
       if (early) continue;
-      __syncthreads();
-      if (k == 0) { d[l] -= p; e[l] = g; e[m] = 0.0; }
-      __syncthreads();
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
     }
   }
   __syncthreads();
@@ -890,14 +917,14 @@ struct EigCtx {
   int matvecs = 0, matvecs64 = 0;
 };
 
-static int apply_h(EigCtx& c, const double* x, double* y, int prec) {
+static int apply_h(EigCtx& c, const void* x, int x_f32, double* y, int prec) {
   c.matvecs++;
   if (prec == TNX_F64) c.matvecs64++;
   if (c.W2)
-    return heff2_matvec(c.L, c.W, c.W2, c.R, x, y, c.chil, c.Dl, c.Dm, c.Dr, c.d, c.chir, prec,
-                        c.k.mv, c.k.mv_bytes, c.s);
-  return heff1_matvec(c.L, c.W, c.R, x, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, c.k.mv,
-                      c.k.mv_bytes, c.s);
+    return heff2_matvec(c.L, c.W, c.W2, c.R, x, x_f32, y, c.chil, c.Dl, c.Dm, c.Dr, c.d, c.chir,
+                        prec, c.k.mv, c.k.mv_bytes, c.s);
+  return heff1_matvec_b(c.L, c.W, c.R, x, x_f32, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, 1, 1,
+                        nullptr, c.N, c.N, c.k.mv, c.k.mv_bytes, c.s);
 }
 
 static int read_status(EigCtx& c, double* host, int n) {
@@ -912,7 +939,7 @@ static int read_status(EigCtx& c, double* host, int n) {
 static int outer_step(EigCtx& c, double* theta, double* rho) {
   KWork& k = c.k;
   const int64_t N = c.N;
-  TNX_TRY(apply_h(c, k.y, k.w, TNX_F64));
+  TNX_TRY(apply_h(c, k.y, 0, k.w, TNX_F64));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, nullptr, true, k.partials, k.h1, N, c.s));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h1, true, k.partials, k.h2, N, c.s));
   TNX_TRY(cgs_pass(k.y, N, 1, k.w, k.h2, false, k.partials, k.h3, N, c.s));
@@ -982,7 +1009,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
 
   // y = v0 / |v0|
   TNX_TRY(cgs_pass(v0, N, 0, const_cast<double*>(v0), nullptr, false, k.partials, k.h3, N, s));
-  hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, v0, k.h3, k.y, N, kOne);
+  hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb), dim3(256), 0, s, v0, k.h3, k.y, N, kOne);
   TNX_LAUNCH_CHECK();
 
   double theta = 0.0, rho = 0.0, anorm = 0.0;
@@ -1014,16 +1041,14 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       break;
     }
     if (c.matvecs >= max_mv) break;
-    // V[0] = y, V[1] = r / rho
-    TNX_TRY(copy_f64(k.y, k.V, N, s));
-    hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, k.V + N, N,
-                       kOne);
-    TNX_LAUNCH_CHECK();
-    // Inner Lanczos cycle in `prec`, from column j = 1.
-    // Inner target. FP64: the final tolerance. Low precision: the first
-    // cycle runs the full thick-restart Lanczos down to the rounding floor of
-    // its arithmetic (it keeps its restart subspace the whole way); later
-    // cycles start from [y, r/|r|] and only need to cut the FP64 residual.
+    // Inner Lanczos cycle in `cprec`, from column j = 1 on V[0] = y,
+    // V[1] = r / rho. Inner target. FP64: the final tolerance. Low
+    // precision: the first cycle runs the full thick-restart Lanczos down to
+    // the rounding floor of its arithmetic (it keeps its restart subspace
+    // the whole way); later cycles start from [y, r/|r|] and only need to
+    // cut the FP64 residual. A low-precision cycle keeps its basis in FP32:
+    // it resolves nothing below FP32 rounding, and the CGS passes and the
+    // matvec read half the bytes.
     const double rho_outer = rho;
     const double unit_lp = cprec == TNX_F32 ? 5.96e-8 : 3.9e-6;
     const double inner_tol =
@@ -1031,83 +1056,112 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                : (first_cycle ? std::max(tol, 4.0 * unit_lp) * anorm
                               : std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer)));
     first_cycle = false;
-    int j = 1;
-    bool inner_done = false;
     have_hy = false;  // the inner cycle reuses k.w
-    while (!inner_done) {
-      double* vj = k.V + (int64_t)j * N;
-      TNX_TRY(apply_h(c, vj, k.w, cprec));
-      const int J = j + 1;
-      TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
-      TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
-      TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
-      hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
-                         k.alpha, k.beta, k.T, kOne);
+    auto cycle = [&](auto* V, auto* Yb) -> int {
+      using VT = std::remove_pointer_t<decltype(V)>;
+      constexpr bool f32 = std::is_same<VT, float>::value;
+      if constexpr (f32) {
+        hipLaunchKernelGGL(to_f32_kernel, dim3(nb), dim3(256), 0, s, k.y, V, N);
+      } else {
+        TNX_TRY(copy_f64(k.y, V, N, s));
+      }
+      hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, V + N, N,
+                         kOne);
       TNX_LAUNCH_CHECK();
-      hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                         k.beta + j, k.s, k.st, kOne);
-      TNX_LAUNCH_CHECK();
-      // V[j+1] = w / beta[j]
-      hipLaunchKernelGGL(normalize_kernel, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
-                         k.V + (int64_t)(j + 1) * N, N, kOne);
-      TNX_LAUNCH_CHECK();
-      double h[4];
-      TNX_TRY(read_status(c, h, 3));
-      const double theta_in = h[0];
-      const double est = h[1];
-      anorm = std::max(anorm, h[2]);
-      const bool exhausted = (J >= N);
-      // FP64 cycles test against the current ||H|| estimate (it grows with
-      // the Ritz values seen); low-precision cycles against their fixed target
-      const double itol = mixed ? inner_tol : tol * anorm;
-      if (est <= itol || exhausted || c.matvecs >= max_mv) {
-        // Ritz vector y = V[0..j] s
-        hipLaunchKernelGGL(vcomb_kernel, dim3(nb), dim3(256), 0, s, k.V, N, J, k.s, k.y, N, kOne);
+      int j = 1;
+      while (true) {
+        TNX_TRY(apply_h(c, V + (int64_t)j * N, f32 ? 1 : 0, k.w, cprec));
+        const int J = j + 1;
+        TNX_TRY(cgs_pass(V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
+        TNX_TRY(cgs_pass(V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
+        TNX_TRY(cgs_pass(V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
+        hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
+                           k.alpha, k.beta, k.T, kOne);
         TNX_LAUNCH_CHECK();
-        inner_done = true;
-        if (!mixed && (est <= tol * anorm || exhausted)) {
-          // FP64 Lanczos: H y = theta y + beta_j s_j V[j+1]
-          hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, k.beta, j,
-                             k.scal, kOne);
+        hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
+                           k.beta + j, k.s, k.st, kOne);
+        TNX_LAUNCH_CHECK();
+        // V[j+1] = w / beta[j]
+        hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
+                           V + (int64_t)(j + 1) * N, N, kOne);
+        TNX_LAUNCH_CHECK();
+        double h[4];
+        TNX_TRY(read_status(c, h, 3));
+        const double theta_in = h[0];
+        const double est = h[1];
+        anorm = std::max(anorm, h[2]);
+        const bool exhausted = (J >= N);
+        // FP64 cycles test against the current ||H|| estimate (it grows with
+        // the Ritz values seen); low-precision cycles against their fixed target
+        const double itol = mixed ? inner_tol : tol * anorm;
+        if (est <= itol || exhausted || c.matvecs >= max_mv) {
+          // Ritz vector y = V[0..j] s
+          hipLaunchKernelGGL(vcomb_kernel<VT>, dim3(nb), dim3(256), 0, s, V, N, J, k.s, k.y, N,
+                             kOne);
           TNX_LAUNCH_CHECK();
-          if (hvec) {
-            hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal, k.y,
-                               exhausted ? nullptr : k.scal + 1, k.V + (int64_t)(j + 1) * N,
-                               hvec, N, kOne, (int64_t)0);
+          if constexpr (f32) {
+            // an FP32 basis is orthonormal only to FP32 rounding: renormalise
+            // y in FP64 before the FP64 step takes its Rayleigh quotient
+            TNX_TRY(cgs_pass(k.y, N, 0, k.y, nullptr, false, k.partials, k.h3, N, s));
+            hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb), dim3(256), 0, s, k.y, k.h3, k.y,
+                               N, kOne);
             TNX_LAUNCH_CHECK();
           }
-          theta = theta_in;
-          rho = est;
-          converged = true;
+          if constexpr (!f32) {
+            if (!mixed && (est <= tol * anorm || exhausted)) {
+              // FP64 Lanczos: H y = theta y + beta_j s_j V[j+1]
+              hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1), dim3(1), 0, s, k.st, k.s, k.beta, j,
+                                 k.scal, kOne);
+              TNX_LAUNCH_CHECK();
+              if (hvec) {
+                hipLaunchKernelGGL(axpbz_dev_kernel, dim3(nb), dim3(256), 0, s, k.scal, k.y,
+                                   exhausted ? nullptr : k.scal + 1, V + (int64_t)(j + 1) * N,
+                                   hvec, N, kOne, (int64_t)0);
+                TNX_LAUNCH_CHECK();
+              }
+              theta = theta_in;
+              rho = est;
+              converged = true;
+            }
+          }
+          return TNX_OK;
         }
-      } else if (J >= mb) {
-        // Thick restart keeping the `keep` lowest Ritz pairs in a basis
-        // where the projected matrix stays tridiagonal.
-        hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                           k.theta, k.S, kMaxBasis, kOne);
-        TNX_LAUNCH_CHECK();
-        int* kp_dev = reinterpret_cast<int*>(k.st + 8);
-        hipLaunchKernelGGL(tr_restart_kernel, dim3(1), dim3(64), 0, s, k.theta, k.S, k.beta, j,
-                           keep, J, k.Cm, k.alpha, k.beta, k.T, kp_dev, kOne);
-        TNX_LAUNCH_CHECK();
-        int kp = 0;
-        TNX_HIP(hipMemcpyAsync(&kp, kp_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-        TNX_HIP(hipStreamSynchronize(s));
-        TNX_CHECK_ARG(kp == keep, "eigsh: restart kept %d of %d vectors", kp, keep);
-        GemmDesc g{};
-        g.M = kp; g.N = (int)N; g.K = J; g.batch = 1;
-        g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = 0; g.a_kcontig = false;
-        g.B = k.V; g.tb = DT::F64; g.ldb = N; g.sB = 0; g.b_ncontig = true;
-        g.C = k.Y; g.tc = DT::F64; g.ldc = N; g.sC = 0;
-        g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
-        TNX_TRY(gemm(g, s));
-        TNX_TRY(copy_f64(k.Y, k.V, (int64_t)kp * N, s));
-        TNX_TRY(copy_f64(k.V + (int64_t)(j + 1) * N, k.V + (int64_t)kp * N, N, s));
-        restarts++;
-        j = kp;
-      } else {
-        j++;
+        if (J >= mb) {
+          // Thick restart keeping the `keep` lowest Ritz pairs in a basis
+          // where the projected matrix stays tridiagonal.
+          hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
+                             k.theta, k.S, kMaxBasis, kOne);
+          TNX_LAUNCH_CHECK();
+          int* kp_dev = reinterpret_cast<int*>(k.st + 8);
+          hipLaunchKernelGGL(tr_restart_kernel, dim3(1), dim3(64), 0, s, k.theta, k.S, k.beta, j,
+                             keep, J, k.Cm, k.alpha, k.beta, k.T, kp_dev, kOne);
+          TNX_LAUNCH_CHECK();
+          int kp = 0;
+          TNX_HIP(hipMemcpyAsync(&kp, kp_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+          TNX_HIP(hipStreamSynchronize(s));
+          TNX_CHECK_ARG(kp == keep, "eigsh: restart kept %d of %d vectors", kp, keep);
+          const DT vt = f32 ? DT::F32 : DT::F64;
+          GemmDesc g{};
+          g.M = kp; g.N = (int)N; g.K = J; g.batch = 1;
+          g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = 0; g.a_kcontig = false;
+          g.B = V; g.tb = vt; g.ldb = N; g.sB = 0; g.b_ncontig = true;
+          g.C = Yb; g.tc = vt; g.ldc = N; g.sC = 0;
+          g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+          TNX_TRY(gemm(g, s));
+          TNX_HIP(hipMemcpyAsync(V, Yb, sizeof(VT) * kp * N, hipMemcpyDeviceToDevice, s));
+          TNX_HIP(hipMemcpyAsync(V + (int64_t)kp * N, V + (int64_t)(j + 1) * N, sizeof(VT) * N,
+                                 hipMemcpyDeviceToDevice, s));
+          restarts++;
+          j = kp;
+        } else {
+          j++;
+        }
       }
+    };
+    if (mixed) {
+      TNX_TRY(cycle(reinterpret_cast<float*>(k.V), reinterpret_cast<float*>(k.Y)));
+    } else {
+      TNX_TRY(cycle(k.V, k.Y));
     }
     if (converged) break;
     if (c.matvecs >= max_mv) break;
@@ -1148,10 +1202,6 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
 // residual estimate meets tol * ||H||_est. One launch per kernel per step
 // covers all problems still iterating.
 // ---------------------------------------------------------------------------
-int heff1_matvec_b(const double* L, const double* W, const double* R, const double* x, double* y,
-                   int chil, int Dl, int Dr, int d, int chir, int prec, int batch, int count,
-                   const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
-                   hipStream_t s);
 size_t heff1_ws_bytes_b(int chil, int Dl, int Dr, int d, int chir, int batch);
 
 // out[i * nst + t] = st[first + t] of launched problem i
@@ -1226,8 +1276,8 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
   };
   auto matvec = [&](const double* x, double* y, int cnt) -> int {
     matvecs++;
-    return heff1_matvec_b(L, W, R, x, y, chil, Dl, Dr, d, chir, TNX_F64, batch, cnt, idx, sp, sp,
-                          mv, mvb, s);
+    return heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, TNX_F64, batch, cnt, idx, sp,
+                          sp, mv, mvb, s);
   };
 
   TNX_TRY(upload(idx, act));
@@ -1238,7 +1288,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
                      pm, N, sp);
   TNX_LAUNCH_CHECK();
   TNX_TRY(cgs_pass(k.y, N, 0, k.y, nullptr, false, k.partials, k.h3, N, s, pm, cnt));
-  hipLaunchKernelGGL(normalize_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.y, k.h3,
+  hipLaunchKernelGGL(normalize_kernel<double>, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.y, k.h3,
                      k.y, N, pm);
   TNX_LAUNCH_CHECK();
   // FP64 step on the start vectors: theta = y.Hy, r = Hy - theta y
@@ -1288,7 +1338,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.y, k.V,
                        N, pm, sp, sp);
     TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(normalize_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.w,
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s, k.w,
                        k.h3 + 1, k.V + N, N, pm);
     TNX_LAUNCH_CHECK();
   }
@@ -1307,7 +1357,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
                        k.beta + j, k.s, k.st, pm);
     TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(normalize_kernel, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
                        k.V + (int64_t)(j + 1) * N, N, pm);
     TNX_LAUNCH_CHECK();
     TNX_TRY(status(0, 3, cnt));
@@ -1338,7 +1388,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
       const int fc = (int)fin.size();
       const int nf = vec_blocks(N, fc);
       PMap pf{fidx, sp};
-      hipLaunchKernelGGL(vcomb_kernel, dim3(nf, fc), dim3(256), 0, s, k.V, N, J, k.s, k.y, N, pf);
+      hipLaunchKernelGGL(vcomb_kernel<double>, dim3(nf, fc), dim3(256), 0, s, k.V, N, J, k.s, k.y, N, pf);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(ritz_scalars_kernel, dim3(1, fc), dim3(1), 0, s, k.st, k.s, k.beta, j,
                          k.scal, pf);
