@@ -65,6 +65,10 @@ def parse():
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--only-batched", action="store_true")
+    p.add_argument("--tiles", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tnpy_amd",
+                                                  "gemm_tiles_mi355x.txt"),
+                   help="GEMM tile table to load (if it exists) so tile choices repeat across runs")
+    p.add_argument("--save-tiles", default=None, help="write the tile table after the run")
     p.add_argument("--kernels-only", action="store_true",
                    help="only the HIP-event kernel rates at chi, plus a plain 256 MiB device copy "
                         "(the reference kernel for scaling HBM counters)")
@@ -306,6 +310,16 @@ def main():
 
     logging.getLogger("tnpy").setLevel(logging.WARNING)
     dev = torch.device("cuda", 0)
+    from tnpy_amd import _lib
+
+    _lib.load_library()
+    if args.tiles and os.path.exists(args.tiles):
+        log(f"GEMM tile table {args.tiles}: {_lib.lib().tnx_tune_load(args.tiles.encode())} entries")
+    if args.save_tiles:
+        import atexit
+
+        atexit.register(lambda: log(f"saved {_lib.lib().tnx_tune_save(args.save_tiles.encode())} tile entries "
+                                    f"to {args.save_tiles}"))
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(args, dev)}), flush=True)
         return

@@ -69,6 +69,15 @@ int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
  * cache. No reference counterpart (quimb leaves this to BLAS). */
 int tnx_set_autotune(int on);
 
+/* Tile table of the autotuner: tnx_tune_save writes every shape -> tile
+ * configuration decided so far in this process to a text file;
+ * tnx_tune_load adds a saved table (returns the number of entries, or a
+ * negative status), so later processes run the same configurations without
+ * timing. TNX_TUNE_CACHE=<file> in the environment loads a table at the
+ * first GEMM. No reference counterpart. */
+int tnx_tune_load(const char* path);
+int tnx_tune_save(const char* path);
+
 /* ---- one-site effective Hamiltonian matvec -----------------------------
  * y[l'][b][r'] = sum L[l][wl][l'] W[wl][wr][k][b] R[r][wr][r'] x[l][k][r]
  * Replaces Environment.one_site_matvec's closure

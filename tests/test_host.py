@@ -117,6 +117,27 @@ class TestABI:
         st = lib.tnx_eigsh_heff1(1, 1, 1, 1, 1, None, 4, 5, 5, 2, 4, C.byref(opts), C.byref(info), None, 0, None)
         assert st == -1 and b"precision" in lib.tnx_last_error()
 
+    def test_tile_table_round_trip(self, lib, tmp_path):
+        """The autotuner's tile table loads valid lines only and saves what it
+        holds (host-only: no GEMM runs)."""
+        src = tmp_path / "tiles.txt"
+        src.write_text(
+            "# comment\n"
+            "0 2048 5120 1024 1 1 1 0 0 0 1 4\n"
+            "1 1024 2048 5120 1 0 1 0 1 0 4 2\n"
+            "0 8 8 8 1 1 1 0 0 0 1 99\n"  # no such configuration: skipped
+            "garbage\n"
+        )
+        assert lib.tnx_tune_load(str(src).encode()) == 2
+        out = tmp_path / "saved.txt"
+        assert lib.tnx_tune_save(str(out).encode()) >= 2
+        lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+        assert "0 2048 5120 1024 1 1 1 0 0 0 1 4" in lines
+        assert "1 1024 2048 5120 1 0 1 0 1 0 4 2" in lines
+        assert lib.tnx_tune_load(str(tmp_path / "missing.txt").encode()) == -1
+        lib.tnx_set_autotune(0)  # clears the table again
+        lib.tnx_set_autotune(1)
+
     def test_product_raises_without_gpu(self):
         import torch
 

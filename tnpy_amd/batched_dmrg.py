@@ -21,7 +21,9 @@ state up to rounding. FP64 throughout.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from itertools import cycle
 from typing import Dict, List, Optional, Sequence
 
@@ -41,6 +43,16 @@ class _Group:
         self.chains = chains
         self.B = len(chains)
         dev = owner.device
+        # each length group runs on its own stream (default priority) with
+        # its own scratch, so groups can be swept concurrently
+        self.stream = torch.cuda.Stream(device=dev)
+        self.stream.wait_stream(torch.cuda.current_stream(dev))
+        self.ws = _lib.Workspace(dev)
+        with torch.cuda.stream(self.stream):
+            self._build(mpos, stacks)
+
+    def _build(self, mpos, stacks: List[torch.Tensor]) -> None:
+        dev = self.owner.device
         self.n = mpos[0].n_sites
         self.W = [
             torch.stack([torch.as_tensor(m.padded[s], device=dev) for m in mpos]).contiguous()
@@ -64,11 +76,11 @@ class _Group:
         l, Dl, Dr, d, r = self.dims(site - 1)
         out = torch.empty(self.B, r, Dr, r, dtype=torch.float64, device=A.device)
         lib = _lib.lib()
-        wp, wb = self.owner._ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        wp, wb = self.ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
         _lib.check(
             lib.tnx_env_left_batched(
                 _lib.ptr(L, "L"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Lout"),
-                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.owner._stream(),
+                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.stream.cuda_stream,
             ),
             "tnx_env_left_batched",
         )
@@ -79,11 +91,11 @@ class _Group:
         l, Dl, Dr, d, r = self.dims(site + 1)
         out = torch.empty(self.B, l, Dl, l, dtype=torch.float64, device=A.device)
         lib = _lib.lib()
-        wp, wb = self.owner._ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        wp, wb = self.ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
         _lib.check(
             lib.tnx_env_right_batched(
                 _lib.ptr(R, "R"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Rout"),
-                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.owner._stream(),
+                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.stream.cuda_stream,
             ),
             "tnx_env_right_batched",
         )
@@ -97,12 +109,12 @@ class _Group:
     def matvec(self, site: int, x: torch.Tensor, y: torch.Tensor) -> None:
         l, Dl, Dr, d, r = self.dims(site)
         lib = _lib.lib()
-        wp, wb = self.owner._ws.get(lib.tnx_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
+        wp, wb = self.ws.get(lib.tnx_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
         _lib.check(
             lib.tnx_heff1_matvec_batched(
                 _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
                 _lib.ptr(x, "x"), _lib.ptr(y, "y"), l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb,
-                self.owner._stream(),
+                self.stream.cuda_stream,
             ),
             "tnx_heff1_matvec_batched",
         )
@@ -123,18 +135,19 @@ class _Group:
             _lib.check(
                 lib.tnx_heff1_dense_batched(
                     _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
-                    _lib.ptr(M, "M"), l, Dl, Dr, d, r, self.B, o._stream(),
+                    _lib.ptr(M, "M"), l, Dl, Dr, d, r, self.B, self.stream.cuda_stream,
                 ),
                 "tnx_heff1_dense_batched",
             )
-            wp, wb = o._ws.get(lib.tnx_syev_lowest_batched_workspace_bytes(N, self.B))
+            wp, wb = self.ws.get(lib.tnx_syev_lowest_batched_workspace_bytes(N, self.B))
             _lib.check(
                 lib.tnx_syev_lowest_batched(_lib.ptr(M, "M"), N, self.B, _lib.ptr(vec, "evec"), evals, wp, wb,
-                                            o._stream()),
+                                            self.stream.cuda_stream),
                 "tnx_syev_lowest_batched",
             )
             self.matvec(site, vec, hvec)
-            o.stats["dense_sites"] += 1
+            with o._lock:
+                o.stats["dense_sites"] += 1
             return np.array(evals[:]), vec, hvec
         opts = _lib.EigshOpts()
         opts.tol = float(tol)
@@ -143,36 +156,37 @@ class _Group:
         opts.precision = _lib.TNX_F64
         opts.keep = int(o._keep)
         infos = (_lib.EigshInfo * self.B)()
-        wp, wb = o._ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, self.B))
+        wp, wb = self.ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, self.B))
         v0 = self.A[site].reshape(self.B, N)
         st = lib.tnx_eigsh_heff1_batched(
             _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
             _lib.ptr(v0, "v0"), _lib.ptr(vec, "vec"), _lib.ptr(hvec, "hvec"), l, Dl, Dr, d, r, self.B,
-            C.byref(opts), infos, wp, wb, o._stream(),
+            C.byref(opts), infos, wp, wb, self.stream.cuda_stream,
         )
         if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
             _lib.check(st, "tnx_eigsh_heff1_batched")
         if st == _lib.TNX_ERR_NOCONV:
             bad = [self.chains[i] for i in range(self.B) if not infos[i].converged]
             logger.warning(f"Lanczos at site {site} hit the matvec limit for chains {bad}.")
-        o.stats["krylov_sites"] += 1
-        o.stats["matvecs"] += sum(infos[i].matvecs for i in range(self.B))
-        o.stats["lockstep_matvecs"] += max(infos[i].matvecs for i in range(self.B))
-        o.stats["restarts"] += max(infos[i].restarts for i in range(self.B))
+        with o._lock:
+            o.stats["krylov_sites"] += 1
+            o.stats["matvecs"] += sum(infos[i].matvecs for i in range(self.B))
+            o.stats["lockstep_matvecs"] += max(infos[i].matvecs for i in range(self.B))
+            o.stats["restarts"] += max(infos[i].restarts for i in range(self.B))
         return np.array([infos[i].eval for i in range(self.B)]), vec, hvec
 
     def gauge(self, site: int, direction: Direction) -> None:
         lib = _lib.lib()
         A = self.A[site]
         _, l, p, r = A.shape
-        wp, wb = self.owner._ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, self.B))
+        wp, wb = self.ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, self.B))
         if direction == Direction.RIGHTWARD:
             nxt = self.A[site + 1]
             out = torch.empty_like(nxt)
             _lib.check(
                 lib.tnx_gauge_right_batched(
                     _lib.ptr(A, "site"), _lib.ptr(nxt, "next"), _lib.ptr(out, "next_out"),
-                    l, p, r, nxt.shape[2] * nxt.shape[3], self.B, wp, wb, self.owner._stream(),
+                    l, p, r, nxt.shape[2] * nxt.shape[3], self.B, wp, wb, self.stream.cuda_stream,
                 ),
                 "tnx_gauge_right_batched",
             )
@@ -183,7 +197,7 @@ class _Group:
             _lib.check(
                 lib.tnx_gauge_left_batched(
                     _lib.ptr(A, "site"), _lib.ptr(prv, "prev"), _lib.ptr(out, "prev_out"),
-                    l, p, r, prv.shape[1] * prv.shape[2], self.B, wp, wb, self.owner._stream(),
+                    l, p, r, prv.shape[1] * prv.shape[2], self.B, wp, wb, self.stream.cuda_stream,
                 ),
                 "tnx_gauge_left_batched",
             )
@@ -196,7 +210,7 @@ class _Group:
         energies, vec, hvec = self.solve(site, tol)
         _lib.check(
             _lib.lib().tnx_axpy(vec.numel(), 1e-5, _lib.ptr(hvec, "hvec"), _lib.ptr(vec, "vec"),
-                                self.owner._stream()),
+                                self.stream.cuda_stream),
             "tnx_axpy",
         )
         self.A[site] = vec.reshape(self.A[site].shape)
@@ -208,13 +222,15 @@ class _Group:
         return energies
 
     def sweep(self, direction: Direction, tol: float) -> np.ndarray:
-        if direction == Direction.LEFTWARD:
-            self.ensure_left(self.n - 1)
-        sites = range(self.n - 1) if direction == Direction.RIGHTWARD else range(self.n - 1, 0, -1)
-        energies = None
-        for site in sites:
-            energies = self.site_step(site, direction, tol)
-        return energies
+        with torch.cuda.stream(self.stream):
+            if direction == Direction.LEFTWARD:
+                self.ensure_left(self.n - 1)
+            sites = range(self.n - 1) if direction == Direction.RIGHTWARD else range(self.n - 1, 0, -1)
+            energies = None
+            for site in sites:
+                energies = self.site_step(site, direction, tol)
+            self.stream.synchronize()
+            return energies
 
 
 class BatchedFiniteDMRG:
@@ -266,6 +282,7 @@ class BatchedFiniteDMRG:
         self._keep = keep
         self._max_matvecs = max_matvecs
         self._ws = _lib.Workspace(self.device)
+        self._lock = threading.Lock()
         self.stats = {"matvecs": 0, "lockstep_matvecs": 0, "restarts": 0, "dense_sites": 0, "krylov_sites": 0}
         nchain = len(self._mpos)
         if mps is not None and len(mps) != nchain:
@@ -292,6 +309,7 @@ class BatchedFiniteDMRG:
                     raise ValueError(f"chains of length {n} have different bond dimensions")
                 stacks = [torch.stack([s[k] for s in starts]).contiguous() for k in range(n)]
             self._groups.append(_Group(self, idx, [self._mpos[i] for i in idx], stacks))
+        torch.cuda.synchronize(self.device)
         self._energies: List[List[float]] = [[] for _ in range(nchain)]
         self.sweep_times: List[float] = []
 
@@ -313,11 +331,19 @@ class BatchedFiniteDMRG:
 
     def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8) -> np.ndarray:
         """One directional sweep of every chain; returns each chain's
-        energy at the last site of the sweep."""
+        energy at the last site of the sweep. Length groups sweep
+        concurrently, one host thread and one stream each (the library
+        calls release the GIL), so one group's small, latency-bound launches
+        overlap another's."""
         out = np.empty(self.n_chains)
-        for g in self._groups:
-            e = g.sweep(direction, tol)
-            out[g.chains] = e
+        if len(self._groups) == 1:
+            g = self._groups[0]
+            out[g.chains] = g.sweep(direction, tol)
+            return out
+        with ThreadPoolExecutor(max_workers=len(self._groups)) as ex:
+            futs = [(g, ex.submit(g.sweep, direction, tol)) for g in self._groups]
+            for g, f in futs:
+                out[g.chains] = f.result()
         return out
 
     def run(self, tol: float = 1e-8, max_sweep: int = 100) -> List[List[float]]:

@@ -19,6 +19,7 @@
 #include "tnx_common.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <unordered_map>
@@ -415,12 +416,45 @@ static std::mutex g_tune_mu;
 static std::unordered_map<TuneKey, int, TuneHash> g_tune;
 static int g_autotune = -1;  // -1: from TNX_AUTOTUNE (default on)
 
+static int tune_load_locked(const char* path);
+
 static bool autotune_on() {
   if (g_autotune < 0) {
     const char* e = getenv("TNX_AUTOTUNE");
     g_autotune = (e && e[0] == '0') ? 0 : 1;
+    // a saved tile table named by TNX_TUNE_CACHE is loaded at first use
+    const char* cache = getenv("TNX_TUNE_CACHE");
+    if (cache && cache[0]) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      tune_load_locked(cache);
+    }
   }
   return g_autotune == 1;
+}
+
+// Tile table file: a comment line, then one line per GEMM shape key,
+// "prec M N K batch a_kcontig b_ncontig ta tb tc splits config".
+static const char* kTuneHeader = "# tnx gemm tile table v1: prec M N K batch ak bn ta tb tc splits cfg";
+
+static int tune_load_locked(const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return -1;
+  char line[256];
+  int n = 0;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    TuneKey k;
+    int cfg;
+    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d", &k.prec, &k.M, &k.N, &k.K, &k.batch,
+               &k.ak, &k.bn, &k.ta, &k.tb, &k.tc, &k.splits, &cfg) != 12)
+      continue;
+    const int ncfg = k.prec == 0 ? Cfgs<0>::n : (k.prec == 1 ? Cfgs<1>::n : Cfgs<2>::n);
+    if (k.prec < 0 || k.prec > 2 || cfg < 0 || cfg >= ncfg) continue;
+    g_tune[k] = cfg;
+    ++n;
+  }
+  fclose(f);
+  return n;
 }
 
 template <int PREC>
@@ -526,11 +560,54 @@ void set_autotune(int on) {
   if (!on) g_tune.clear();
 }
 
+int tune_load(const char* path) {
+  autotune_on();  // resolve the environment first so it cannot override later
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const int n = tune_load_locked(path);
+  if (n < 0) set_error("tnx_tune_load: cannot read %s", path);
+  return n < 0 ? TNX_ERR_INVALID : n;
+}
+
+int tune_save(const char* path) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  FILE* f = fopen(path, "w");
+  if (!f) {
+    set_error("tnx_tune_save: cannot write %s", path);
+    return TNX_ERR_INVALID;
+  }
+  fprintf(f, "%s\n", kTuneHeader);
+  int n = 0;
+  for (const auto& kv : g_tune) {
+    const TuneKey& k = kv.first;
+    fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.M, k.N, k.K, k.batch, k.ak, k.bn,
+            k.ta, k.tb, k.tc, k.splits, kv.second);
+    ++n;
+  }
+  fclose(f);
+  return n;
+}
+
 }  // namespace tnx
 
 extern "C" int tnx_set_autotune(int on) {
   tnx::set_autotune(on);
   return TNX_OK;
+}
+
+extern "C" int tnx_tune_load(const char* path) {
+  if (!path) {
+    tnx::set_error("tnx_tune_load: null path");
+    return TNX_ERR_INVALID;
+  }
+  return tnx::tune_load(path);
+}
+
+extern "C" int tnx_tune_save(const char* path) {
+  if (!path) {
+    tnx::set_error("tnx_tune_save: null path");
+    return TNX_ERR_INVALID;
+  }
+  return tnx::tune_save(path);
 }
 
 extern "C" int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,

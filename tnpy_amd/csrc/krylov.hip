@@ -712,13 +712,26 @@ static size_t eigsh_ws_bytes_n(size_t N, size_t mvb, int max_basis) {
   return b + 256;
 }
 
+// Single-problem solver: the Lanczos block plus the FP64 correction block
+// of the mixed-precision path (W = H V for up to m vectors, and the small
+// arrays of its dense Rayleigh-Ritz).
+static size_t gd_ws_bytes(size_t N, int m) {
+  return align256((size_t)m * N * 8) + align256((size_t)8 * (kMaxBasis + 8) * 8);
+}
+
+static size_t eigsh_single_ws_bytes(size_t N, size_t mvb, int max_basis) {
+  const int m = std::max(3, std::min(max_basis, kMaxBasis));
+  return eigsh_ws_bytes_n(N, mvb, max_basis) + gd_ws_bytes(N, m);
+}
+
 size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
-  return eigsh_ws_bytes_n((size_t)chil * d * chir, heff1_ws_bytes(chil, Dl, Dr, d, chir), max_basis);
+  return eigsh_single_ws_bytes((size_t)chil * d * chir, heff1_ws_bytes(chil, Dl, Dr, d, chir),
+                               max_basis);
 }
 
 size_t eigsh2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir, int max_basis) {
-  return eigsh_ws_bytes_n((size_t)chil * d * d * chir, heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir),
-                          max_basis);
+  return eigsh_single_ws_bytes((size_t)chil * d * d * chir,
+                               heff2_ws_bytes(chil, Dl, Dm, Dr, d, chir), max_basis);
 }
 
 static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
@@ -915,6 +928,8 @@ struct EigCtx {
   hipStream_t s;
   KWork k;
   int matvecs = 0, matvecs64 = 0;
+  double* Wd = nullptr;   // FP64 correction: H V (m x N)
+  double* gsm = nullptr;  // its small arrays (8 x (kMaxBasis + 8))
 };
 
 static int apply_h(EigCtx& c, const void* x, int x_f32, double* y, int prec) {
@@ -957,6 +972,166 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                      const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
                      size_t ws_bytes, size_t mvb);
 
+// ---------------------------------------------------------------------------
+// FP64 correction after a low-precision Lanczos cycle (mixed precision).
+// A cycle that has reached its arithmetic's floor holds, in its kept Ritz
+// vectors, the cluster of low eigenvalues its Krylov space has resolved. A
+// fresh FP64 Lanczos from [y, r] would have to resolve that cluster again.
+// Instead: take the `keep` lowest Ritz vectors of the cycle, orthonormalise
+// them in FP64, apply H to each in FP64 (W = H V), and run a subspace
+// expansion with an exact dense projection G = V^T H V: each step takes
+// the lowest Ritz pair of G, its FP64 residual r = W s - theta V s (exact,
+// no recurrence estimate), stops when |r| <= tol ||H||_est, else adds
+// r (orthogonalised) to V and H r to W. (Without a preconditioner this
+// spans the same space as Lanczos from the kept block, but keeps the block
+// exact in FP64.)
+// ---------------------------------------------------------------------------
+
+// z = V s, hz = W s, r = hz - theta z (theta = *st), J vectors each
+__global__ void gd_residual_kernel(const double* __restrict__ V, const double* __restrict__ Wm,
+                                   int64_t ld, int J, const double* __restrict__ sv,
+                                   const double* __restrict__ st, double* __restrict__ z,
+                                   double* __restrict__ hz, double* __restrict__ r, int64_t N) {
+  __shared__ double cs[kMaxBasis + 1];
+  if ((int)threadIdx.x < J) cs[threadIdx.x] = sv[threadIdx.x];
+  __syncthreads();
+  const double th = st[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < J; ++j) {
+      a += cs[j] * V[(int64_t)j * ld + i];
+      b += cs[j] * Wm[(int64_t)j * ld + i];
+    }
+    z[i] = a;
+    hz[i] = b;
+    r[i] = b - th * a;
+  }
+}
+
+// column and row J of the dense projected matrix: G[i][J] = G[J][i] = h[i]
+__global__ void gd_gcol_kernel(const double* __restrict__ h, int J, double* __restrict__ G) {
+  const int i = threadIdx.x;
+  if (i <= J) {
+    G[i * kMaxBasis + J] = h[i];
+    G[J * kMaxBasis + i] = h[i];
+  }
+}
+
+// compact symmetric copy A[J][J] of G (row stride kMaxBasis)
+__global__ void gd_compact_kernel(const double* __restrict__ G, int J, double* __restrict__ A) {
+  for (int e = threadIdx.x; e < J * J; e += blockDim.x) {
+    const int i = e / J, j = e % J;
+    A[e] = 0.5 * (G[i * kMaxBasis + j] + G[j * kMaxBasis + i]);
+  }
+}
+
+// Returns TNX_OK; *done says whether the criterion was met (then k.y = z,
+// hvec = H z, *theta and *rho set). If not, k.y holds the best Ritz vector
+// and the caller continues with FP64 Lanczos from it.
+static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, double tol,
+                         double* anorm, int max_mv, double* hvec, double* theta, double* rho,
+                         bool* done) {
+  KWork& k = c.k;
+  hipStream_t s = c.s;
+  const int64_t N = c.N;
+  const int nb = vec_blocks(N);
+  *done = false;
+  const int kk = std::max(1, std::min(keep, J32));
+  double* A = k.S;        // compact copy of G for the dense solve
+  double* Vst = k.Cm;     // its Householder vectors
+  double* tau = c.gsm;
+  double* dg = tau + (kMaxBasis + 8);
+  double* eg = dg + (kMaxBasis + 8);
+  double* zt = eg + (kMaxBasis + 8);
+  double* stg = zt + (kMaxBasis + 8);
+  double* G = k.T;
+  // kept Ritz vectors of the low-precision cycle, in FP64: Y = V32 S[:, :kk]
+  hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J32, k.theta,
+                     k.S, kMaxBasis, kOne);
+  TNX_LAUNCH_CHECK();
+  {
+    GemmDesc g{};
+    g.M = kk; g.N = (int)N; g.K = J32; g.batch = 1;
+    g.A = k.S; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = 0; g.a_kcontig = false;
+    g.B = V32; g.tb = DT::F32; g.ldb = N; g.sB = 0; g.b_ncontig = true;
+    g.C = k.Y; g.tc = DT::F64; g.ldc = N; g.sC = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+    TNX_TRY(gemm(g, s));
+  }
+  double* V = k.V;  // the FP64 basis of the correction (V32 is no longer needed)
+  TNX_HIP(hipMemcpyAsync(V, k.Y, sizeof(double) * kk * N, hipMemcpyDeviceToDevice, s));
+  // orthonormalise (CGS2) and apply H in FP64, building G column by column
+  for (int i = 0; i < kk; ++i) {
+    double* vi = V + (int64_t)i * N;
+    if (i > 0) {
+      TNX_TRY(cgs_pass(V, N, i, vi, nullptr, true, k.partials, k.h1, N, s));
+      TNX_TRY(cgs_pass(V, N, i, vi, k.h1, true, k.partials, k.h2, N, s));
+    }
+    TNX_TRY(cgs_pass(V, N, i, vi, i > 0 ? k.h2 : nullptr, false, k.partials, k.h3, N, s));
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb), dim3(256), 0, s, vi, k.h3 + i, vi, N,
+                       kOne);
+    TNX_LAUNCH_CHECK();
+    double* wi = c.Wd + (int64_t)i * N;
+    TNX_TRY(apply_h(c, vi, 0, wi, TNX_F64));
+    TNX_TRY(cgs_pass(V, N, i + 1, wi, nullptr, true, k.partials, k.h1, N, s));
+    hipLaunchKernelGGL(gd_gcol_kernel, dim3(1), dim3(kMaxBasis + 1), 0, s, k.h1, i, G);
+    TNX_LAUNCH_CHECK();
+  }
+  int J = kk;
+  while (true) {
+    // lowest Ritz pair of G (J x J): Householder tridiagonalisation,
+    // multisection + inverse iteration, back-transform into k.s
+    hipLaunchKernelGGL(gd_compact_kernel, dim3(1), dim3(256), 0, s, G, J, A);
+    TNX_LAUNCH_CHECK();
+    if (J == 1) {
+      TNX_TRY(fill_f64(k.s, 1, 1.0, s));
+      TNX_HIP(hipMemcpyAsync(stg, A, sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, J, Vst, tau, dg, eg, kOne);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, J,
+                         (const double*)nullptr, zt, stg, kOne);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, J, zt, k.s, kOne,
+                         (int64_t)0);
+      TNX_LAUNCH_CHECK();
+    }
+    // z = V s -> k.y, H z = W s -> k.w, r -> V[J] (the next basis slot)
+    double* r = V + (int64_t)J * N;
+    hipLaunchKernelGGL(gd_residual_kernel, dim3(nb), dim3(256), 0, s, V, c.Wd, N, J, k.s, stg,
+                       k.y, k.w, r, N);
+    TNX_LAUNCH_CHECK();
+    TNX_TRY(cgs_pass(r, N, 0, r, nullptr, false, k.partials, k.h3, N, s));
+    double h[2];
+    TNX_HIP(hipMemcpyAsync(&h[0], stg, sizeof(double), hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipMemcpyAsync(&h[1], k.h3, sizeof(double), hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipStreamSynchronize(s));
+    *theta = h[0];
+    *rho = std::sqrt(std::max(h[1], 0.0));
+    *anorm = std::max(*anorm, std::fabs(h[0]));
+    if (*rho <= tol * *anorm || *rho == 0.0) {
+      if (hvec) TNX_TRY(copy_f64(k.w, hvec, N, s));
+      *done = true;
+      return TNX_OK;
+    }
+    if (J >= m || c.matvecs >= max_mv) return TNX_OK;
+    // expand: r orthogonalised against V, normalised, and H r
+    TNX_TRY(cgs_pass(V, N, J, r, nullptr, true, k.partials, k.h1, N, s));
+    TNX_TRY(cgs_pass(V, N, J, r, k.h1, true, k.partials, k.h2, N, s));
+    TNX_TRY(cgs_pass(V, N, J, r, k.h2, false, k.partials, k.h3, N, s));
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb), dim3(256), 0, s, r, k.h3 + J, r, N,
+                       kOne);
+    TNX_LAUNCH_CHECK();
+    double* wj = c.Wd + (int64_t)J * N;
+    TNX_TRY(apply_h(c, r, 0, wj, TNX_F64));
+    TNX_TRY(cgs_pass(V, N, J + 1, wj, nullptr, true, k.partials, k.h1, N, s));
+    hipLaunchKernelGGL(gd_gcol_kernel, dim3(1), dim3(kMaxBasis + 1), 0, s, k.h1, J, G);
+    TNX_LAUNCH_CHECK();
+    ++J;
+  }
+}
+
 int eigsh_heff1(const double* L, const double* W, const double* R, const double* v0,
                 double* vec, double* hvec, int chil, int Dl, int Dr, int d, int chir,
                 const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info, void* ws,
@@ -994,9 +1169,17 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
   TNX_CHECK_ARG(opts->precision >= TNX_F64 && opts->precision <= TNX_BF16X3, "eigsh: bad precision");
   TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh: negative tol");
   const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
-  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_ws_bytes_n((size_t)c.N, mvb, m), "eigsh: workspace too small");
+  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_single_ws_bytes((size_t)c.N, mvb, m),
+                "eigsh: workspace too small");
   hipStream_t s = c.s;
   carve(c.k, ws, m, (size_t)c.N, mvb);
+  {
+    // the correction block follows the Lanczos block
+    char* gp = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255) +
+               (eigsh_ws_bytes_n((size_t)c.N, mvb, m) - 256);
+    c.Wd = (double*)gp;
+    c.gsm = (double*)(gp + align256((size_t)m * c.N * 8));
+  }
   KWork& k = c.k;
   const int64_t N = c.N;
   const int prec = opts->precision;
@@ -1055,8 +1238,12 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         !mixed ? tol * anorm
                : (first_cycle ? std::max(tol, 4.0 * unit_lp) * anorm
                               : std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer)));
+    // a first low-precision cycle that stops at its floor, with a target
+    // below that floor, hands its kept block to the FP64 correction
+    const bool gd_after = mixed && first_cycle && tol < 8.0 * unit_lp;
     first_cycle = false;
     have_hy = false;  // the inner cycle reuses k.w
+    int last_J = 0;   // Lanczos basis size when the cycle stopped
     auto cycle = [&](auto* V, auto* Yb) -> int {
       using VT = std::remove_pointer_t<decltype(V)>;
       constexpr bool f32 = std::is_same<VT, float>::value;
@@ -1095,6 +1282,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         // the Ritz values seen); low-precision cycles against their fixed target
         const double itol = mixed ? inner_tol : tol * anorm;
         if (est <= itol || exhausted || c.matvecs >= max_mv) {
+          last_J = J;
           // Ritz vector y = V[0..j] s
           hipLaunchKernelGGL(vcomb_kernel<VT>, dim3(nb), dim3(256), 0, s, V, N, J, k.s, k.y, N,
                              kOne);
@@ -1162,6 +1350,17 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       TNX_TRY(cycle(reinterpret_cast<float*>(k.V), reinterpret_cast<float*>(k.Y)));
     } else {
       TNX_TRY(cycle(k.V, k.Y));
+    }
+    if (gd_after && last_J > 0 && c.matvecs < max_mv) {
+      bool done = false;
+      TNX_TRY(gd_correction(c, reinterpret_cast<const float*>(k.V), last_J, keep, mb, tol, &anorm,
+                            max_mv, hvec, &theta, &rho, &done));
+      if (done) {
+        converged = true;
+        have_hy = false;  // outputs: k.y, and hvec written by the correction
+        break;
+      }
+      cprec = TNX_F64;  // continue with FP64 Lanczos from the best vector
     }
     if (converged) break;
     if (c.matvecs >= max_mv) break;
