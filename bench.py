@@ -65,6 +65,10 @@ def parse():
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--only-batched", action="store_true")
+    p.add_argument("--two-site", type=int, default=0, metavar="K",
+                   help="only time K two-site bond updates at the middle of a random-field Heisenberg "
+                        "chain (n=--two-site-n, h=0.5, chi=--chi) in FP64 and in the mixed precision")
+    p.add_argument("--two-site-n", type=int, default=128)
     p.add_argument("--tiles", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tnpy_amd",
                                                   "gemm_tiles_mi355x.txt"),
                    help="GEMM tile table to load (if it exists) so tile choices repeat across runs")
@@ -85,6 +89,60 @@ def copy_reference(mib=256, reps=10):
     ms = event_time(lambda: y.copy_(x), reps)
     return {"bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8, "ms": ms,
             "GBps": 2 * x.numel() * 8 / ms / 1e6}
+
+
+def two_site_leg(args, dev):
+    """Two-site DMRG sample: random-field Heisenberg chain (Delta=1, h_i ~
+    U(-0.5, 0.5) from the seed), seeded random MPS at chi, orthogonality
+    centre moved to the middle by QR gauge moves (the state is unchanged),
+    then K rightward two-site updates (two-site Lanczos, truncation of the
+    (chi*d) x (d*chi) block back to chi, environment update) timed in FP64
+    and in the mixed precision from the same state."""
+    import torch
+
+    from tnpy_amd import Direction, FiniteDMRG, MatrixProductState, _lib
+    from tnpy_amd.model import RandomHeisenberg
+
+    n, chi, K = args.two_site_n, args.chi, args.two_site
+    model = RandomHeisenberg(n=n, h=0.5, seed=args.seed)
+    mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=args.seed, device=dev, host_rng=False)
+    mid = max(0, n // 2 - K // 2)
+    ws = _lib.Workspace(dev)
+    for site in range(mid):
+        mps.split_tensor(site, Direction.RIGHTWARD, workspace=ws)
+    snap = [t.clone() for t in mps.tensors]
+    del mps
+    torch.cuda.synchronize()
+    out = {}
+    for prec in ("fp64", args.mixed):
+        d = FiniteDMRG(model.mpo, bond_dim=chi, block_size=2, mps=MatrixProductState([t.clone() for t in snap], device=dev),
+                       precision=prec, max_basis=args.max_basis, keep=args.keep, compute_variance=False, device=dev)
+        d.environment.ensure_left(mid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        energies = []
+        for site in range(mid, min(mid + K, n - 1)):
+            energies.append(d._two_site_step(site, Direction.RIGHTWARD, args.tol))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[prec] = {"seconds": dt, "bonds": len(energies), "s_per_bond": dt / max(1, len(energies)),
+                     "matvecs": d.stats["matvecs"], "matvecs_f64": d.stats["matvecs_f64"],
+                     "energies": energies}
+        log(f"two-site {prec}: {len(energies)} bonds in {dt:.2f}s ({dt / max(1, len(energies)):.3f} s/bond), "
+            f"matvecs {d.stats['matvecs']} (f64 {d.stats['matvecs_f64']}), E={energies[-1]:.12f}")
+        del d
+        torch.cuda.empty_cache()
+    e64, emx = out["fp64"]["energies"], out[args.mixed]["energies"]
+    return {
+        "workload": (f"random-field Heisenberg n={n} h=0.5 chi={chi}, {K} two-site updates at the chain middle "
+                     f"(sites {mid}..{mid + K - 1}), tol={args.tol}"),
+        "fp64": {k: v for k, v in out["fp64"].items() if k != "energies"},
+        "mixed": {k: v for k, v in out[args.mixed].items() if k != "energies"},
+        "precision": args.mixed,
+        "abs_dE_first_bond": abs(e64[0] - emx[0]),
+        "abs_dE_last_bond": abs(e64[-1] - emx[-1]),
+        "speedup_vs_fp64": out["fp64"]["seconds"] / out[args.mixed]["seconds"],
+    }
 
 
 def batched_leg(args, dev):
@@ -322,6 +380,9 @@ def main():
                                     f"to {args.save_tiles}"))
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(args, dev)}), flush=True)
+        return
+    if args.two_site > 0:
+        print(json.dumps({"two_site": two_site_leg(args, dev)}), flush=True)
         return
     if args.kernels_only:
         rates = kernel_rates(args.chi, 5, 2, ["fp64", args.mixed])

@@ -87,33 +87,47 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const VT* __restrict__ V,
   for (int j = 0; j < JMAX; ++j) acc[j] = 0.0;
   double nrm = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-    double vv[JMAX];
-    if constexpr (std::is_same<VT, double>::value) {
+  if constexpr (std::is_same<VT, double>::value) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+      double vv[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? V[(int64_t)j * ldv + i] : 0.0;
-    } else {
-      // FP32 basis: unconditional loads (rows past J re-read row J-1, a cache
-      // hit) and the widening after all of them. A guarded load with its
-      // conversion in the same branch waits for its data before the next
-      // load issues.
-      VT raw[JMAX];
+      double wi = w[i];
+      if (hsub) {
+#pragma unroll
+        for (int j = 0; j < JMAX; ++j) wi -= vv[j] * hs[j];
+        w[i] = wi;
+      }
+      if (DOTS) {
+#pragma unroll
+        for (int j = 0; j < JMAX; ++j) acc[j] += vv[j] * wi;
+      }
+      nrm += wi * wi;
+    }
+  } else {
+    // FP32 basis: unconditional loads (rows past J re-read row J-1, a cache
+    // hit) and the widening after all of them. A guarded load with its
+    // conversion in the same branch waits for its data before the next
+    // load issues.
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+      float raw[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) raw[j] = V[(int64_t)(j < J ? j : J - 1) * ldv + i];
+      double vv[JMAX];
 #pragma unroll
       for (int j = 0; j < JMAX; ++j) vv[j] = (j < J) ? (double)raw[j] : 0.0;
-    }
-    double wi = w[i];
-    if (hsub) {
+      double wi = w[i];
+      if (hsub) {
 #pragma unroll
-      for (int j = 0; j < JMAX; ++j) wi -= vv[j] * hs[j];
-      w[i] = wi;
-    }
-    if (DOTS) {
+        for (int j = 0; j < JMAX; ++j) wi -= vv[j] * hs[j];
+        w[i] = wi;
+      }
+      if (DOTS) {
 #pragma unroll
-      for (int j = 0; j < JMAX; ++j) acc[j] += vv[j] * wi;
+        for (int j = 0; j < JMAX; ++j) acc[j] += vv[j] * wi;
+      }
+      nrm += wi * wi;
     }
-    nrm += wi * wi;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (DOTS) {
@@ -1037,7 +1051,19 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
   const int64_t N = c.N;
   const int nb = vec_blocks(N);
   *done = false;
-  const int kk = std::max(1, std::min(keep, J32));
+  // kept Ritz values of the cycle (ascending); the block is the cluster
+  // within 1% of ||H||_est of the lowest (at least 2, at most `keep`): a
+  // gapped spectrum needs a couple of FP64 matvecs here, a clustered one
+  // keeps the whole cluster
+  hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J32, k.theta,
+                     k.S, kMaxBasis, kOne);
+  TNX_LAUNCH_CHECK();
+  std::vector<double> th(J32);
+  TNX_HIP(hipMemcpyAsync(th.data(), k.theta, sizeof(double) * J32, hipMemcpyDeviceToHost, s));
+  TNX_HIP(hipStreamSynchronize(s));
+  int kk = 1;
+  while (kk < std::min(keep, J32) && th[kk] - th[0] <= 0.01 * *anorm) ++kk;
+  kk = std::min(std::max(kk, 2), std::min(keep, J32));
   double* A = k.S;        // compact copy of G for the dense solve
   double* Vst = k.Cm;     // its Householder vectors
   double* tau = c.gsm;
@@ -1047,9 +1073,6 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
   double* stg = zt + (kMaxBasis + 8);
   double* G = k.T;
   // kept Ritz vectors of the low-precision cycle, in FP64: Y = V32 S[:, :kk]
-  hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J32, k.theta,
-                     k.S, kMaxBasis, kOne);
-  TNX_LAUNCH_CHECK();
   {
     GemmDesc g{};
     g.M = kk; g.N = (int)N; g.K = J32; g.batch = 1;
@@ -1234,13 +1257,17 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
     // matvec read half the bytes.
     const double rho_outer = rho;
     const double unit_lp = cprec == TNX_F32 ? 5.96e-8 : 3.9e-6;
+    // A first low-precision cycle whose target is below its arithmetic's
+    // floor runs on its own (low-precision operator) residual estimate down
+    // to tol: the residual it reaches against H stays at the floor, but its
+    // Krylov space keeps growing into the low spectrum (near-degenerate
+    // states below the first converged Ritz value appear late), and its kept
+    // block then goes to the FP64 correction.
+    const bool gd_after = mixed && first_cycle && tol < 8.0 * unit_lp;
     const double inner_tol =
         !mixed ? tol * anorm
-               : (first_cycle ? std::max(tol, 4.0 * unit_lp) * anorm
+               : (first_cycle ? (gd_after ? tol : std::max(tol, 4.0 * unit_lp)) * anorm
                               : std::max(tol * anorm, std::min(1e-3 * rho_outer, 0.5 * rho_outer)));
-    // a first low-precision cycle that stops at its floor, with a target
-    // below that floor, hands its kept block to the FP64 correction
-    const bool gd_after = mixed && first_cycle && tol < 8.0 * unit_lp;
     first_cycle = false;
     have_hy = false;  // the inner cycle reuses k.w
     int last_J = 0;   // Lanczos basis size when the cycle stopped
@@ -1346,12 +1373,12 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         }
       }
     };
-    if (mixed) {
+    if (mixed && N % 2 == 0) {
       TNX_TRY(cycle(reinterpret_cast<float*>(k.V), reinterpret_cast<float*>(k.Y)));
     } else {
       TNX_TRY(cycle(k.V, k.Y));
     }
-    if (gd_after && last_J > 0 && c.matvecs < max_mv) {
+    if (gd_after && N % 2 == 0 && last_J > 0 && c.matvecs < max_mv) {
       bool done = false;
       TNX_TRY(gd_correction(c, reinterpret_cast<const float*>(k.V), last_J, keep, mb, tol, &anorm,
                             max_mv, hvec, &theta, &rho, &done));
