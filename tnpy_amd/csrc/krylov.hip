@@ -148,6 +148,91 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const VT* __restrict__ V,
   if (threadIdx.x == 0) out[J] = red[0][JMAX] + red[1][JMAX] + red[2][JMAX] + red[3][JMAX];
 }
 
+// The same pass with two lanes per element pair: lane h = lane & 1 of a
+// pair reads basis rows [h*JMAX/2, (h+1)*JMAX/2) as 8-byte (float2) or
+// 16-byte (double2) loads of the pair's two elements; the subtraction's
+// partial sums are combined across the pair with one shuffle, and each
+// lane accumulates dot products for its own rows (reduced over the lanes of
+// its parity). Wide loads and half the live registers per lane of the
+// one-lane-per-element kernel. Needs N and ldv even and V, w aligned to
+// two elements (cgs_pass checks).
+template <int JMAX, bool DOTS, typename VT>
+__global__ __launch_bounds__(256) void cgs_pair_kernel(const VT* __restrict__ V, int64_t ldv,
+                                                       int J, double* __restrict__ w,
+                                                       const double* __restrict__ hsub,
+                                                       double* __restrict__ partials, int64_t N,
+                                                       PMap pm) {
+  using P2 = typename std::conditional<std::is_same<VT, float>::value, float2, double2>::type;
+  constexpr int JH = JMAX / 2;
+  {
+    const int64_t o = pm.off();
+    V += o; w += o; partials += o;
+    if (hsub) hsub += o;
+  }
+  __shared__ double hs[JMAX];
+  __shared__ double red[4][JMAX + 1];
+  if (threadIdx.x < JMAX) hs[threadIdx.x] = (hsub && (int)threadIdx.x < J) ? hsub[threadIdx.x] : 0.0;
+  __syncthreads();
+  const int h = threadIdx.x & 1;
+  const int r0 = h * JH;
+  double acc[JH];
+#pragma unroll
+  for (int j = 0; j < JH; ++j) acc[j] = 0.0;
+  double nrm = 0.0;
+  const P2* V2 = reinterpret_cast<const P2*>(V);
+  double2* w2 = reinterpret_cast<double2*>(w);
+  const int64_t N2 = N / 2, ld2 = ldv / 2;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 1;
+  for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 1; e < N2; e += stride) {
+    P2 raw[JH];
+#pragma unroll
+    for (int jj = 0; jj < JH; ++jj) {
+      const int j = r0 + jj;
+      raw[jj] = V2[(int64_t)(j < J ? j : J - 1) * ld2 + e];
+    }
+    const double2 wp = w2[e];
+    double w0 = wp.x, w1 = wp.y;
+    if (hsub) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < JH; ++jj) {
+        const double hj = hs[r0 + jj];
+        s0 += (double)raw[jj].x * hj;
+        s1 += (double)raw[jj].y * hj;
+      }
+      s0 += __shfl_xor(s0, 1, 64);
+      s1 += __shfl_xor(s1, 1, 64);
+      w0 -= s0;
+      w1 -= s1;
+      if (h == 0) w2[e] = make_double2(w0, w1);
+    }
+    if (DOTS) {
+#pragma unroll
+      for (int jj = 0; jj < JH; ++jj) acc[jj] += (double)raw[jj].x * w0 + (double)raw[jj].y * w1;
+    }
+    if (h == 0) nrm += w0 * w0 + w1 * w1;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (DOTS) {
+#pragma unroll
+    for (int jj = 0; jj < JH; ++jj) {
+      double t = acc[jj];
+#pragma unroll
+      for (int o = 32; o > 1; o >>= 1) t += __shfl_xor(t, o, 64);  // lanes of one parity
+      if (lane < 2) red[wv][lane * JH + jj] = t;
+    }
+  }
+  {
+    const double t = wave_sum(nrm);
+    if (lane == 0) red[wv][JMAX] = t;
+  }
+  __syncthreads();
+  double* out = partials + (int64_t)blockIdx.x * kPartStride;
+  if ((int)threadIdx.x < J && DOTS)
+    out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+  if (threadIdx.x == 0) out[J] = red[0][JMAX] + red[1][JMAX] + red[2][JMAX] + red[3][JMAX];
+}
+
 // out[j] = sum over blocks of partials[blk][j], j <= J. One workgroup per
 // j; a fixed strided split and a fixed tree, so the sum is deterministic.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* partials, int nblk,
@@ -174,11 +259,21 @@ static int cgs_pass(const VT* V, int64_t ldv, int J, double* w, const double* hs
                     PMap pm = kOne, int count = 1) {
   const int nblk = (int)std::min<int64_t>(count > 1 ? std::max(kCgsBlocks / count, 8) : kCgsBlocks,
                                           (N + 255) / 256);
+  // Passes with dot products take the pair kernel (measured faster); the
+  // subtract-and-norm pass keeps one lane per element (the pair form is
+  // slower there). The pair kernel needs even lengths and pointers aligned
+  // to two elements (offsets of batched problems are multiples of 256 bytes).
+  const bool pair = dots && J >= 2 && N % 2 == 0 && ldv % 2 == 0 &&
+                    ((uintptr_t)V % (2 * sizeof(VT))) == 0 && ((uintptr_t)w % 16) == 0;
   auto go = [&](auto jtag, auto dtag) {
     constexpr int JM = decltype(jtag)::value;
     constexpr bool DT_ = decltype(dtag)::value;
-    hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_, VT>), dim3(nblk, count), dim3(256), 0, s, V, ldv,
-                       J, w, hsub, partials, N, pm);
+    if (pair)
+      hipLaunchKernelGGL((cgs_pair_kernel<JM, DT_, VT>), dim3(nblk, count), dim3(256), 0, s, V,
+                         ldv, J, w, hsub, partials, N, pm);
+    else
+      hipLaunchKernelGGL((cgs_pass_kernel<JM, DT_, VT>), dim3(nblk, count), dim3(256), 0, s, V, ldv,
+                         J, w, hsub, partials, N, pm);
   };
   using T8 = std::integral_constant<int, 8>;
   using T16 = std::integral_constant<int, 16>;
