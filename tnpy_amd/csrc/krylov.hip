@@ -148,22 +148,30 @@ __global__ __launch_bounds__(256) void cgs_pass_kernel(const VT* __restrict__ V,
   if (threadIdx.x == 0) out[J] = red[0][JMAX] + red[1][JMAX] + red[2][JMAX] + red[3][JMAX];
 }
 
-// The same pass with two lanes per element pair: lane h = lane & 1 of a
-// pair reads basis rows [h*JMAX/2, (h+1)*JMAX/2) as 8-byte (float2) or
-// 16-byte (double2) loads of the pair's two elements; the subtraction's
-// partial sums are combined across the pair with one shuffle, and each
-// lane accumulates dot products for its own rows (reduced over the lanes of
-// its parity). Wide loads and half the live registers per lane of the
-// one-lane-per-element kernel. Needs N and ldv even and V, w aligned to
-// two elements (cgs_pass checks).
+// The same pass with R lanes per group of E elements (FP64 basis: R = 2,
+// E = 2; FP32: R = 4, E = 4; one 16-byte load per basis row): lane
+// h = lane % R of a group reads basis rows [h*JMAX/R, (h+1)*JMAX/R) of its
+// group; the subtraction's partial sums are combined across the group by
+// shuffles, and each lane accumulates dot products for its own rows
+// (reduced over the lanes with its h). Every load moves 16 bytes and a lane
+// holds 1/R of the rows, against 4- or 8-byte loads of all rows for one
+// lane per element. Needs N and ldv multiples of E and V, w 16-byte aligned
+// (cgs_pass checks).
+__device__ __forceinline__ double vcomp(const double2& v, int q) { return q == 0 ? v.x : v.y; }
+__device__ __forceinline__ double vcomp(const float4& v, int q) {
+  return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
+
 template <int JMAX, bool DOTS, typename VT>
 __global__ __launch_bounds__(256) void cgs_pair_kernel(const VT* __restrict__ V, int64_t ldv,
                                                        int J, double* __restrict__ w,
                                                        const double* __restrict__ hsub,
                                                        double* __restrict__ partials, int64_t N,
                                                        PMap pm) {
-  using P2 = typename std::conditional<std::is_same<VT, float>::value, float2, double2>::type;
-  constexpr int JH = JMAX / 2;
+  using PV = typename std::conditional<std::is_same<VT, float>::value, float4, double2>::type;
+  constexpr int E = 16 / sizeof(VT);
+  constexpr int R = E;
+  constexpr int JH = JMAX / R;
   {
     const int64_t o = pm.off();
     V += o; w += o; partials += o;
@@ -173,58 +181,78 @@ __global__ __launch_bounds__(256) void cgs_pair_kernel(const VT* __restrict__ V,
   __shared__ double red[4][JMAX + 1];
   if (threadIdx.x < JMAX) hs[threadIdx.x] = (hsub && (int)threadIdx.x < J) ? hsub[threadIdx.x] : 0.0;
   __syncthreads();
-  const int h = threadIdx.x & 1;
+  const int h = threadIdx.x & (R - 1);
   const int r0 = h * JH;
   double acc[JH];
 #pragma unroll
   for (int j = 0; j < JH; ++j) acc[j] = 0.0;
   double nrm = 0.0;
-  const P2* V2 = reinterpret_cast<const P2*>(V);
+  const PV* Vp = reinterpret_cast<const PV*>(V);
   double2* w2 = reinterpret_cast<double2*>(w);
-  const int64_t N2 = N / 2, ld2 = ldv / 2;
-  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 1;
-  for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 1; e < N2; e += stride) {
-    P2 raw[JH];
+  const int64_t NE = N / E, ldE = ldv / E;
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) / R;
+  for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / R; e < NE; e += stride) {
+    PV raw[JH];
 #pragma unroll
     for (int jj = 0; jj < JH; ++jj) {
       const int j = r0 + jj;
-      raw[jj] = V2[(int64_t)(j < J ? j : J - 1) * ld2 + e];
+      raw[jj] = Vp[(int64_t)(j < J ? j : J - 1) * ldE + e];
     }
-    const double2 wp = w2[e];
-    double w0 = wp.x, w1 = wp.y;
+    double wv[E];
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+      const double2 t = w2[e * (E / 2) + q];
+      wv[2 * q] = t.x;
+      wv[2 * q + 1] = t.y;
+    }
     if (hsub) {
-      double s0 = 0.0, s1 = 0.0;
+      double sm[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q) sm[q] = 0.0;
 #pragma unroll
       for (int jj = 0; jj < JH; ++jj) {
         const double hj = hs[r0 + jj];
-        s0 += (double)raw[jj].x * hj;
-        s1 += (double)raw[jj].y * hj;
+#pragma unroll
+        for (int q = 0; q < E; ++q) sm[q] += vcomp(raw[jj], q) * hj;
       }
-      s0 += __shfl_xor(s0, 1, 64);
-      s1 += __shfl_xor(s1, 1, 64);
-      w0 -= s0;
-      w1 -= s1;
-      if (h == 0) w2[e] = make_double2(w0, w1);
+#pragma unroll
+      for (int o = 1; o < R; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < E; ++q) sm[q] += __shfl_xor(sm[q], o, 64);
+#pragma unroll
+      for (int q = 0; q < E; ++q) wv[q] -= sm[q];
+      if (h == 0) {
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) w2[e * (E / 2) + q] = make_double2(wv[2 * q], wv[2 * q + 1]);
+      }
     }
     if (DOTS) {
 #pragma unroll
-      for (int jj = 0; jj < JH; ++jj) acc[jj] += (double)raw[jj].x * w0 + (double)raw[jj].y * w1;
+      for (int jj = 0; jj < JH; ++jj) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < E; ++q) t += vcomp(raw[jj], q) * wv[q];
+        acc[jj] += t;
+      }
     }
-    if (h == 0) nrm += w0 * w0 + w1 * w1;
+    if (h == 0) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) nrm += wv[q] * wv[q];
+    }
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv_ = threadIdx.x >> 6;
   if (DOTS) {
 #pragma unroll
     for (int jj = 0; jj < JH; ++jj) {
       double t = acc[jj];
 #pragma unroll
-      for (int o = 32; o > 1; o >>= 1) t += __shfl_xor(t, o, 64);  // lanes of one parity
-      if (lane < 2) red[wv][lane * JH + jj] = t;
+      for (int o = 32; o >= R; o >>= 1) t += __shfl_xor(t, o, 64);  // lanes with one h
+      if (lane < R) red[wv_][lane * JH + jj] = t;
     }
   }
   {
     const double t = wave_sum(nrm);
-    if (lane == 0) red[wv][JMAX] = t;
+    if (lane == 0) red[wv_][JMAX] = t;
   }
   __syncthreads();
   double* out = partials + (int64_t)blockIdx.x * kPartStride;
@@ -261,10 +289,12 @@ static int cgs_pass(const VT* V, int64_t ldv, int J, double* w, const double* hs
                                           (N + 255) / 256);
   // Passes with dot products take the pair kernel (measured faster); the
   // subtract-and-norm pass keeps one lane per element (the pair form is
-  // slower there). The pair kernel needs even lengths and pointers aligned
-  // to two elements (offsets of batched problems are multiples of 256 bytes).
-  const bool pair = dots && J >= 2 && N % 2 == 0 && ldv % 2 == 0 &&
-                    ((uintptr_t)V % (2 * sizeof(VT))) == 0 && ((uintptr_t)w % 16) == 0;
+  // slower there). The pair kernel needs lengths in whole 16-byte groups and
+  // 16-byte aligned pointers (offsets of batched problems are multiples of
+  // 256 bytes).
+  constexpr int E = 16 / sizeof(VT);
+  const bool pair = dots && J >= 2 && N % E == 0 && ldv % E == 0 && ((uintptr_t)V % 16) == 0 &&
+                    ((uintptr_t)w % 16) == 0;
   auto go = [&](auto jtag, auto dtag) {
     constexpr int JM = decltype(jtag)::value;
     constexpr bool DT_ = decltype(dtag)::value;
@@ -1147,7 +1177,7 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
   const int nb = vec_blocks(N);
   *done = false;
   // kept Ritz values of the cycle (ascending); the block is the cluster
-  // within 1% of ||H||_est of the lowest (at least 2, at most `keep`): a
+  // within 0.1% of ||H||_est of the lowest (at least 2, at most `keep`): a
   // gapped spectrum needs a couple of FP64 matvecs here, a clustered one
   // keeps the whole cluster
   hipLaunchKernelGGL(tridiag_all_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J32, k.theta,
@@ -1157,7 +1187,7 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
   TNX_HIP(hipMemcpyAsync(th.data(), k.theta, sizeof(double) * J32, hipMemcpyDeviceToHost, s));
   TNX_HIP(hipStreamSynchronize(s));
   int kk = 1;
-  while (kk < std::min(keep, J32) && th[kk] - th[0] <= 0.01 * *anorm) ++kk;
+  while (kk < std::min(keep, J32) && th[kk] - th[0] <= 1e-3 * *anorm) ++kk;
   kk = std::min(std::max(kk, 2), std::min(keep, J32));
   double* A = k.S;        // compact copy of G for the dense solve
   double* Vst = k.Cm;     // its Householder vectors
