@@ -358,6 +358,13 @@ __global__ void normalize_kernel(const double* __restrict__ x, const double* __r
     out[i] = (OT)(x[i] * inv);
 }
 
+// out = (double)x
+__global__ void to_f64_kernel(const float* __restrict__ x, double* __restrict__ out, int64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)x[i];
+}
+
 // out = (float)x (the first vector of a low-precision Krylov basis)
 __global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ out, int64_t N) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -1409,7 +1416,16 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       TNX_LAUNCH_CHECK();
       int j = 1;
       while (true) {
-        TNX_TRY(apply_h(c, V + (int64_t)j * N, f32 ? 1 : 0, k.w, cprec));
+        if constexpr (f32) {
+          // the matvec reads FP64 input faster than FP32 (16-byte vs 8-byte
+          // tile loads): widen V[j] into k.y, which is free until the cycle's
+          // Ritz vector is formed
+          hipLaunchKernelGGL(to_f64_kernel, dim3(nb), dim3(256), 0, s, V + (int64_t)j * N, k.y, N);
+          TNX_LAUNCH_CHECK();
+          TNX_TRY(apply_h(c, k.y, 0, k.w, cprec));
+        } else {
+          TNX_TRY(apply_h(c, V + (int64_t)j * N, 0, k.w, cprec));
+        }
         const int J = j + 1;
         TNX_TRY(cgs_pass(V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
         TNX_TRY(cgs_pass(V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
