@@ -301,12 +301,13 @@ def measured_traffic(precision, M, N, K):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_stage1_chi1024_traffic.json")
     try:
         with open(path) as f:
-            t = json.load(f)
+            data = json.load(f)
     except OSError:
         return None
-    if (t.get("precision"), t.get("M"), t.get("N"), t.get("K")) != (precision, M, N, K):
-        return None
-    return t["bytes_per_launch"]
+    for t in data.get("entries", [data]):
+        if (t.get("precision"), t.get("M"), t.get("N"), t.get("K")) == (precision, M, N, K):
+            return t["bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(chi, D, d, n, mv_per_site, bond, reps):
@@ -518,7 +519,7 @@ def main():
             "peak": PEAK_TFLOPS[args.mixed],
             "unit": "TFLOP/s",
             "frac": r["gemm_tflops"] / PEAK_TFLOPS[args.mixed],
-            "traffic": None,
+            "traffic": measured_traffic(args.mixed, 2 * chi, 5 * chi, chi),
         },
         "roofline_fp64": {
             "bound": "mfma",
