@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--only-batched", action="store_true")
+    p.add_argument("--batched-serial", action="store_true",
+                   help="sweep the chain-length groups one after another instead of concurrently")
     p.add_argument("--two-site", type=int, default=0, metavar="K",
                    help="only time K two-site bond updates at the middle of a random-field Heisenberg "
                         "chain (n=--two-site-n, h=0.5, chi=--chi) in FP64 and in the mixed precision")
@@ -167,7 +169,8 @@ def batched_leg(args, dev):
     t = time.perf_counter()
     bd = BatchedFiniteDMRG([m.mpo for m in models], chi=chi,
                            seeds=[args.batch_seed * 100003 + i for i in range(len(ns))],
-                           max_basis=args.max_basis, keep=args.keep, device=dev)
+                           max_basis=args.max_basis, keep=args.keep, concurrent=not args.batched_serial,
+                           device=dev)
     torch.cuda.synchronize()
     log(f"batched: {len(ns)} chains {dict(sorted((n, ns.count(n)) for n in set(ns)))} chi={chi} "
         f"ready in {time.perf_counter() - t:.1f}s")

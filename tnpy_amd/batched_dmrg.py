@@ -246,6 +246,8 @@ class BatchedFiniteDMRG:
             each drawn as ``MatrixProductState.random(n, chi, d, seed)``.
         exact_solver_dim, max_basis, keep: as for ``FiniteDMRG``.
         max_matvecs: Lanczos matvec limit per site.
+        concurrent: sweep the length groups concurrently (one host thread
+            and stream each); False runs them one after another.
     """
 
     def __init__(
@@ -260,6 +262,7 @@ class BatchedFiniteDMRG:
         max_basis: int = 64,
         keep: int = 20,
         max_matvecs: int = 5000,
+        concurrent: bool = True,
         device: Optional[torch.device] = None,
     ):
         if chi is not None and bond_dim is not None and chi != bond_dim:
@@ -281,6 +284,7 @@ class BatchedFiniteDMRG:
         self._max_basis = max_basis
         self._keep = keep
         self._max_matvecs = max_matvecs
+        self._concurrent = concurrent
         self._ws = _lib.Workspace(self.device)
         self._lock = threading.Lock()
         self.stats = {"matvecs": 0, "lockstep_matvecs": 0, "restarts": 0, "dense_sites": 0, "krylov_sites": 0}
@@ -336,9 +340,9 @@ class BatchedFiniteDMRG:
         calls release the GIL), so one group's small, latency-bound launches
         overlap another's."""
         out = np.empty(self.n_chains)
-        if len(self._groups) == 1:
-            g = self._groups[0]
-            out[g.chains] = g.sweep(direction, tol)
+        if len(self._groups) == 1 or not self._concurrent:
+            for g in self._groups:
+                out[g.chains] = g.sweep(direction, tol)
             return out
         with ThreadPoolExecutor(max_workers=len(self._groups)) as ex:
             futs = [(g, ex.submit(g.sweep, direction, tol)) for g in self._groups]
