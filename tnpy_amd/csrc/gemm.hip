@@ -1,22 +1,10 @@
 // Strided, batched GEMM on CDNA4 matrix cores, the contraction engine of
 // every tensor operation in the library (H_eff matvec stages, environment
-// updates, QR trailing updates, gauge absorption).
-//
-// One kernel template, three arithmetics:
-//   F64    : v_mfma_f64_16x16x4_f64, FP64 LDS tiles [BK][BM]
-//   F32    : v_mfma_f32_16x16x4_f32, FP32 LDS tiles [BK][BM]
-//   BF16X3 : v_mfma_f32_16x16x32_bf16 three times per tile pair,
-//            a*b ~ ahi*bhi + ahi*blo + alo*bhi with a = ahi + alo (bf16),
-//            FP32 accumulate, LDS tiles [BM][BK] so each lane reads its
-//            8 consecutive k values with one 16-byte LDS read.
-// Tile configurations (block tile BM x BN, WGM x WGN wavefronts of 64 lanes,
-// K-tile depth BK) are listed in Cfgs below; the autotuner picks one per
-// shape. Each wave owns a (BM/WGM) x (BN/WGN) sub-tile of 16x16 MFMA
-// accumulators. Operands may be FP64 or FP32 in HBM (runtime, wave-uniform
-// branch); they are converted while staging the tile in LDS. Global loads of
-// tile t+1 are issued before the MFMAs of tile t and their results are first
-// used after them; LDS is double-buffered (one barrier per k-tile).
-#include "tnx_common.h"
+// updates, QR trailing updates, gauge absorption): host side. The kernel
+// template and tile configurations are in gemm_kernel.h; this file holds the
+// split-K rule, the choice between typed and generic staging, the autotuner
+// and the tile table.
+#include "gemm_kernel.h"
 
 #include <cmath>
 #include <cstdio>
@@ -26,257 +14,6 @@
 
 
 namespace tnx {
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-struct GemmArgs {
-  int M, N, K;
-  const void* A; int64_t lda, sA; int ta;
-  const void* B; int64_t ldb, sB; int tb;
-  void* C; int64_t ldc, sC; int tc;
-  double alpha, beta;
-  int tiles_n;  // number of tiles along N
-  int kc;       // K extent of one split (== K when not split)
-  double* partial;  // split-K partial sums [batch][split][M][N], or null
-  int nb2;          // second batch level (1 = none)
-  int64_t sA2, sB2, sC2;
-  const int* bidx;  // launched -> logical batch index, or null
-};
-
-template <int PREC> struct PT;
-template <> struct PT<0> { using lds_t = double; using acc_t = f64x4; };
-template <> struct PT<1> { using lds_t = float;  using acc_t = f32x4; };
-template <> struct PT<2> { using lds_t = __bf16; using acc_t = f32x4; };
-
-// WGM x WGN wavefronts per block; each wave owns a (BM/WGM) x (BN/WGN) tile.
-template <int PREC, int BM, int BN, bool A_K, bool B_N, int WGM, int WGN, int BK>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
-  using lds_t = typename PT<PREC>::lds_t;
-  using acc_t = typename PT<PREC>::acc_t;
-  constexpr int NT = 64 * WGM * WGN;
-  constexpr int EA = BM * BK / NT;
-  constexpr int EB = BK * BN / NT;
-  constexpr int WM = BM / WGM, WN = BN / WGN;
-  constexpr int TM = WM / 16, TN = WN / 16;
-  static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile not divisible by the block");
-  // LDS geometry. F64/F32: [BK][BM + PADM]. BF16X3: [BM][BK + PADK] for
-  // hi and lo parts (the k index is innermost so a lane's 8 values are
-  // one 16-byte read).
-  constexpr int PADM = (PREC == 0) ? 2 : 4;
-  constexpr int PADK = 8;
-  constexpr int A_ELEMS = (PREC == 2) ? 2 * BM * (BK + PADK) : BK * (BM + PADM);
-  constexpr int B_ELEMS = (PREC == 2) ? 2 * BN * (BK + PADK) : BK * (BN + PADM);
-  // 16-byte alignment: the BF16X3 path reads 8 bf16 values per lane at once
-  __shared__ __attribute__((aligned(16))) lds_t As[2][A_ELEMS];
-  __shared__ __attribute__((aligned(16))) lds_t Bs[2][B_ELEMS];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm0 = (wave / WGN) * WM;
-  const int wn0 = (wave % WGN) * WN;
-
-  const int bat = g.bidx ? g.bidx[blockIdx.y] : (int)blockIdx.y;
-  const int tile = blockIdx.x;
-  const int split = blockIdx.z;
-  const int kbeg = split * g.kc;
-  const int kend = min(g.K, kbeg + g.kc);
-  const int m0 = (tile / g.tiles_n) * BM;
-  const int n0 = (tile % g.tiles_n) * BN;
-
-  const int b1 = bat / g.nb2, b2 = bat % g.nb2;
-  const char* Ab = (const char*)g.A + ((size_t)b1 * g.sA + (size_t)b2 * g.sA2) * (g.ta == 0 ? 8 : 4);
-  const char* Bb = (const char*)g.B + ((size_t)b1 * g.sB + (size_t)b2 * g.sB2) * (g.tb == 0 ? 8 : 4);
-
-  // Staging registers hold the raw loaded bits (a double, or a float in the
-  // low word) so that no instruction uses a global load's result until
-  // store_tiles, after the MFMAs of the current k-block: the loads of
-  // block k+1 stay in flight under the compute of block k.
-  uint64_t ua[EA], ub[EB];
-
-  auto load_tiles = [&](int k0) {
-#pragma unroll
-    for (int e = 0; e < EA; ++e) {
-      const int idx = tid + e * NT;
-      int mi, ki;
-      if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
-      const int m = m0 + mi, k = k0 + ki;
-      uint64_t v = 0;
-      if (m < g.M && k < kend) {
-        const int64_t off = A_K ? (int64_t)m * g.lda + k : (int64_t)k * g.lda + m;
-        if (g.ta == 0) v = reinterpret_cast<const uint64_t*>(Ab)[off];
-        else v = reinterpret_cast<const uint32_t*>(Ab)[off];
-      }
-      ua[e] = v;
-    }
-#pragma unroll
-    for (int e = 0; e < EB; ++e) {
-      const int idx = tid + e * NT;
-      int ni, ki;
-      if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
-      const int n = n0 + ni, k = k0 + ki;
-      uint64_t v = 0;
-      if (n < g.N && k < kend) {
-        const int64_t off = B_N ? (int64_t)k * g.ldb + n : (int64_t)n * g.ldb + k;
-        if (g.tb == 0) v = reinterpret_cast<const uint64_t*>(Bb)[off];
-        else v = reinterpret_cast<const uint32_t*>(Bb)[off];
-      }
-      ub[e] = v;
-    }
-  };
-  auto as_double = [](uint64_t v, int t) -> double {
-    return t == 0 ? __longlong_as_double((long long)v) : (double)__uint_as_float((uint32_t)v);
-  };
-  auto as_float = [](uint64_t v, int t) -> float {
-    return t == 0 ? (float)__longlong_as_double((long long)v) : __uint_as_float((uint32_t)v);
-  };
-
-  auto store_tiles = [&](int buf) {
-#pragma unroll
-    for (int e = 0; e < EA; ++e) {
-      const int idx = tid + e * NT;
-      int mi, ki;
-      if (A_K) { ki = idx % BK; mi = idx / BK; } else { mi = idx % BM; ki = idx / BM; }
-      if constexpr (PREC == 0) {
-        As[buf][ki * (BM + PADM) + mi] = as_double(ua[e], g.ta);
-      } else if constexpr (PREC == 1) {
-        As[buf][ki * (BM + PADM) + mi] = as_float(ua[e], g.ta);
-      } else {
-        const float x = as_float(ua[e], g.ta);
-        const __bf16 hi = (__bf16)x;
-        const __bf16 lo = (__bf16)(x - (float)hi);
-        As[buf][mi * (BK + PADK) + ki] = hi;
-        As[buf][BM * (BK + PADK) + mi * (BK + PADK) + ki] = lo;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < EB; ++e) {
-      const int idx = tid + e * NT;
-      int ni, ki;
-      if (B_N) { ni = idx % BN; ki = idx / BN; } else { ki = idx % BK; ni = idx / BK; }
-      if constexpr (PREC == 0) {
-        Bs[buf][ki * (BN + PADM) + ni] = as_double(ub[e], g.tb);
-      } else if constexpr (PREC == 1) {
-        Bs[buf][ki * (BN + PADM) + ni] = as_float(ub[e], g.tb);
-      } else {
-        const float x = as_float(ub[e], g.tb);
-        const __bf16 hi = (__bf16)x;
-        const __bf16 lo = (__bf16)(x - (float)hi);
-        Bs[buf][ni * (BK + PADK) + ki] = hi;
-        Bs[buf][BN * (BK + PADK) + ni * (BK + PADK) + ki] = lo;
-      }
-    }
-  };
-
-  acc_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-  const int nk = (kend - kbeg + BK - 1) / BK;
-  load_tiles(kbeg);
-  store_tiles(0);
-  __syncthreads();
-
-  const int lr = lane & 15;   // row (A) / column (B) inside a 16x16 tile
-  const int lk = lane >> 4;   // k group of the lane
-
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * BK);
-    const lds_t* as = As[buf];
-    const lds_t* bs = Bs[buf];
-    if constexpr (PREC == 0 || PREC == 1) {
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        lds_t a[TM], b[TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i) a[i] = as[(kk + lk) * (BM + PADM) + wm0 + i * 16 + lr];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) b[j] = bs[(kk + lk) * (BN + PADM) + wn0 + j * 16 + lr];
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            if constexpr (PREC == 0)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-            else
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-          }
-      }
-    } else {
-      bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
-      constexpr int LO_A = BM * (BK + PADK);
-      constexpr int LO_B = BN * (BK + PADK);
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int r = wm0 + i * 16 + lr;
-        ah[i] = *reinterpret_cast<const bf16x8*>(&as[r * (BK + PADK) + lk * 8]);
-        al[i] = *reinterpret_cast<const bf16x8*>(&as[LO_A + r * (BK + PADK) + lk * 8]);
-      }
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int c = wn0 + j * 16 + lr;
-        bh[j] = *reinterpret_cast<const bf16x8*>(&bs[c * (BK + PADK) + lk * 8]);
-        bl[j] = *reinterpret_cast<const bf16x8*>(&bs[LO_B + c * (BK + PADK) + lk * 8]);
-      }
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        }
-    }
-    if (kt + 1 < nk) store_tiles(buf ^ 1);
-    __syncthreads();
-  }
-
-  // Epilogue. 16x16 accumulator tile, lane group lk = lane/16, item v:
-  //   f32 MFMAs: row = 4*lk + v ;  f64 MFMA: row = lk + 4*v ; column = lane%16
-  auto row_of = [&](int v) { return (PREC == 0) ? (lk + 4 * v) : (lk * 4 + v); };
-  if (g.partial) {
-    double* P = g.partial + ((size_t)blockIdx.y * gridDim.z + split) * (size_t)g.M * g.N;
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int m = m0 + wm0 + i * 16 + row_of(v);
-          const int n = n0 + wn0 + j * 16 + lr;
-          if (m < g.M && n < g.N) P[(size_t)m * g.N + n] = (double)acc[i][j][v];
-        }
-    return;
-  }
-  char* Cb = (char*)g.C + ((size_t)b1 * g.sC + (size_t)b2 * g.sC2) * (g.tc == 0 ? 8 : 4);
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int m = m0 + wm0 + i * 16 + row_of(v);
-        const int n = n0 + wn0 + j * 16 + lr;
-        if (m < g.M && n < g.N) {
-          const int64_t off = (int64_t)m * g.ldc + n;
-          double val = g.alpha * (double)acc[i][j][v];
-          if (g.tc == 0) {
-            double* c = reinterpret_cast<double*>(Cb) + off;
-            if (g.beta != 0.0) val += g.beta * *c;
-            *c = val;
-          } else {
-            float* c = reinterpret_cast<float*>(Cb) + off;
-            if (g.beta != 0.0) val += g.beta * (double)*c;
-            *c = (float)val;
-          }
-        }
-      }
-}
 
 // C = alpha * sum_z partial[bat][z] + beta * C   (fixed summation order)
 __global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, int M, int N,
@@ -308,60 +45,47 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, i
 // ---------------------------------------------------------------------------
 // Tile configurations, split-K rule and the runtime autotuner.
 //
-// A config only changes how output tiles are cut and scheduled: every output
-// element still accumulates its K range in order, so results are bitwise
-// identical whichever config runs. Split-K does change the rounding and is
-// therefore decided by a fixed rule on the shape, never by timing.
+// A config only changes how output tiles are cut and scheduled (results are
+// bitwise identical whichever config runs). Split-K does change the rounding
+// and is therefore decided by a fixed rule on the shape, never by timing.
 // ---------------------------------------------------------------------------
-struct Cfg { int bm, bn, wgm, wgn, bk; };
-
-template <int PREC> struct Cfgs;
-template <> struct Cfgs<0> {
-  static constexpr int n = 6;
-  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 16},
-                               {128, 256, 2, 4, 16}, {256, 160, 4, 2, 16}, {160, 256, 2, 4, 16}};
-};
-template <> struct Cfgs<1> {
-  static constexpr int n = 6;
-  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 32},
-                               {128, 256, 2, 4, 32}, {256, 160, 4, 2, 32}, {160, 256, 2, 4, 32}};
-};
-template <> struct Cfgs<2> {
-  static constexpr int n = 6;
-  static constexpr Cfg c[n] = {{64, 64, 2, 2, 32}, {128, 128, 2, 2, 32}, {256, 128, 4, 2, 32},
-                               {128, 256, 2, 4, 32}, {256, 160, 4, 2, 32}, {160, 256, 2, 4, 32}};
-};
-constexpr Cfg Cfgs<0>::c[];
-constexpr Cfg Cfgs<1>::c[];
-constexpr Cfg Cfgs<2>::c[];
-
-template <int PREC, int BM, int BN, int WGM, int WGN, int BK>
-static void launch_cfg(const GemmDesc& d, const GemmArgs& g0, int splits, hipStream_t s) {
-  GemmArgs g = g0;
-  g.tiles_n = ceil_div(d.N, BN);
-  const int tiles = ceil_div(d.M, BM) * g.tiles_n;
-  dim3 grid(tiles, d.batch, splits);
-  dim3 block(64 * WGM * WGN);
-  if (d.a_kcontig && d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, true, WGM, WGN, BK>), grid, block, 0, s, g);
-  else if (d.a_kcontig && !d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, true, false, WGM, WGN, BK>), grid, block, 0, s, g);
-  else if (!d.a_kcontig && d.b_ncontig)
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, true, WGM, WGN, BK>), grid, block, 0, s, g);
-  else
-    hipLaunchKernelGGL((gemm_kernel<PREC, BM, BN, false, false, WGM, WGN, BK>), grid, block, 0, s, g);
+template <int PREC>
+static void launch_index(int ci, const GemmDesc& d, const GemmArgs& g, int splits, bool typed,
+                         hipStream_t s) {
+  if (PREC == 0) gemm_launch_f64(ci, d, g, splits, typed, s);
+  else if (PREC == 1) gemm_launch_f32(ci, d, g, splits, typed, s);
+  else gemm_launch_bf16(ci, d, g, splits, typed, s);
 }
 
-template <int PREC>
-static void launch_index(int ci, const GemmDesc& d, const GemmArgs& g, int splits, hipStream_t s) {
-#define TNX_CFG(I)                                                                              \
-  if (ci == I && I < Cfgs<PREC>::n) {                                                           \
-    constexpr Cfg c = Cfgs<PREC>::c[I < Cfgs<PREC>::n ? I : 0];                                 \
-    launch_cfg<PREC, c.bm, c.bn, c.wgm, c.wgn, c.bk>(d, g, splits, s);                          \
-    return;                                                                                     \
-  }
-  TNX_CFG(0) TNX_CFG(1) TNX_CFG(2) TNX_CFG(3) TNX_CFG(4) TNX_CFG(5) TNX_CFG(6)
-#undef TNX_CFG
+// One operand of the typed staging: contiguous extent `ext` (elements),
+// leading dimension, batch strides and base all in whole 16-byte vectors,
+// one batch entry (rows x ld, contiguous extent on the last row) < 2 GiB.
+static bool typed_operand_ok(const void* p, DT t, int64_t ld, int64_t s1, int64_t s2, int nb2,
+                             int64_t ext, int64_t rows, int bk_rows) {
+  const int64_t es = t == DT::F64 ? 8 : 4, v = 16 / es;
+  if (((uintptr_t)p & 15) != 0 || ext % v != 0 || ld % v != 0) return false;
+  if ((s1 * es) % 16 != 0 || (nb2 > 1 && (s2 * es) % 16 != 0)) return false;
+  const int64_t span = ((rows - 1) * ld + ext) * es;
+  const int64_t lim = ((int64_t)1 << 31) - 64;
+  return span < lim && (int64_t)bk_rows * ld * es < lim;
+}
+
+bool gemm_typed_ok(const GemmDesc& d, int kc) {
+  const bool f64f64 = d.ta == DT::F64 && d.tb == DT::F64;
+  const bool f64f32 = d.ta == DT::F64 && d.tb == DT::F32;
+  const bool f32f64 = d.ta == DT::F32 && d.tb == DT::F64;
+  if (!(f64f64 || f64f32 || (d.prec != Prec::F64 && f32f64))) return false;
+  const int nb2 = d.nb2 > 0 ? d.nb2 : 1;
+  // the K range of a split is a whole number of vectors when K-contiguous
+  if (d.a_kcontig && kc % (d.ta == DT::F64 ? 2 : 4) != 0) return false;
+  if (!d.b_ncontig && kc % (d.tb == DT::F64 ? 2 : 4) != 0) return false;
+  const bool a_ok = d.a_kcontig
+      ? typed_operand_ok(d.A, d.ta, d.lda, d.sA, d.sA2, nb2, d.K, d.M, 32)
+      : typed_operand_ok(d.A, d.ta, d.lda, d.sA, d.sA2, nb2, d.M, d.K, 32);
+  const bool b_ok = d.b_ncontig
+      ? typed_operand_ok(d.B, d.tb, d.ldb, d.sB, d.sB2, nb2, d.N, d.K, 32)
+      : typed_operand_ok(d.B, d.tb, d.ldb, d.sB, d.sB2, nb2, d.K, d.N, 32);
+  return a_ok && b_ok;
 }
 
 // Deterministic split-K rule: split only when the output is small (fewer
@@ -477,6 +201,7 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
     g.partial = d.splitk_ws;
   }
   const int nsplit = splits > 1 ? ceil_div(d.K, g.kc) : 1;
+  const bool typed = gemm_typed_ok(d, g.kc);
   const TuneKey key{PREC, d.M, d.N, d.K, d.batch, d.a_kcontig, d.b_ncontig, (int)d.ta, (int)d.tb,
                     (int)d.tc, nsplit};
   int ci = -1;
@@ -507,10 +232,10 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
         hipEventCreate(&e1);
         float best = 1e30f;
         for (int i = 0; i < Cfgs<PREC>::n; ++i) {
-          launch_index<PREC>(i, d, gt, nsplit, s);  // warm
+          launch_index<PREC>(i, d, gt, nsplit, typed, s);  // warm
           hipEventRecord(e0, s);
-          launch_index<PREC>(i, d, gt, nsplit, s);
-          launch_index<PREC>(i, d, gt, nsplit, s);
+          launch_index<PREC>(i, d, gt, nsplit, typed, s);
+          launch_index<PREC>(i, d, gt, nsplit, typed, s);
           hipEventRecord(e1, s);
           hipEventSynchronize(e1);
           float ms = 0.f;
@@ -527,7 +252,7 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = ci;
   }
-  launch_index<PREC>(ci, d, g, nsplit, s);
+  launch_index<PREC>(ci, d, g, nsplit, typed, s);
   TNX_LAUNCH_CHECK();
   if (g.partial) {
     const int64_t MN = (int64_t)d.M * d.N;
