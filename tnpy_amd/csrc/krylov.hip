@@ -453,6 +453,175 @@ __device__ int sturm_count(const double* a, const double* b2, int n, double x) {
   return cnt;
 }
 
+// Lane i's value of a wave-distributed double, as a wave-uniform scalar
+// (v_readlane: no LDS round trip). i must be wave-uniform.
+__device__ __forceinline__ double lane_val(double v, int i) {
+  const long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), i);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(u >> 32), i);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int lane_int(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
+
+// sturm_count with row k of the tridiagonal held by lane k (n <= 64): the
+// same recurrence, rescaling and zero convention, unrolled by four so the
+// rescale is not a per-step select, and the sign test without branches
+// (the loop issues few instructions per step; one wave runs it alone).
+__device__ int sturm_count_reg(double ar, double b2r, int n, double x) {
+  double pm = 1.0, p = lane_val(ar, 0) - x;
+  bool neg_prev = false;
+  int cnt = 0;
+  auto sign_step = [&](double v) {
+    const bool neg = (v < 0.0) | ((v == 0.0) & !neg_prev);
+    cnt += (int)(neg ^ neg_prev);
+    neg_prev = neg;
+  };
+  auto rec = [&](int i) {  // p_{i+1} from p_i, p_{i-1}
+    const double pn = fma(lane_val(ar, i + 1) - x, p, -lane_val(b2r, i) * pm);
+    pm = p;
+    p = pn;
+  };
+  sign_step(p);
+  int i = 0;
+  for (; i + 4 < n; i += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      rec(i + u);
+      sign_step(p);
+    }
+    const int e = __builtin_amdgcn_frexp_exp(p == 0.0 ? pm : p);
+    p = ldexp(p, -e);
+    pm = ldexp(pm, -e);
+  }
+  for (; i + 1 < n; ++i) {
+    rec(i);
+    sign_step(p);
+  }
+  return cnt;
+}
+
+// The lowest eigenpair for n <= 64 (every Lanczos projection): the same
+// steps as the LDS form in tridiag_lowest_kernel, with row k of T and of the
+// factors in lane k's registers. The serial recurrences (Sturm counts,
+// factorisation, the two solves) run on all lanes as wave-uniform scalar
+// chains that read a row with v_readlane and write lane k's value with a
+// lane select, so no step waits on LDS.
+__device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* __restrict__ bd,
+                                   int n, const double* __restrict__ bres, double* __restrict__ s,
+                                   double* __restrict__ st) {
+  const int lane = threadIdx.x;
+  const double ar = lane < n ? ad[lane] : 0.0;
+  const double br = lane < n - 1 ? bd[lane] : 0.0;
+  const double b2r = br * br;
+  // Gershgorin bounds and ||T|| estimate
+  double lo, hi, tnorm;
+  {
+    // the shuffle runs on every lane (a lane that is switched off reads as
+    // zero to the lanes that shuffle from it); lane 0 then drops its value
+    const double bup = __shfl_up(br, 1, 64);
+    const double bl = lane > 0 ? fabs(bup) : 0.0;
+    const double r = bl + (lane < n - 1 ? fabs(br) : 0.0);
+    lo = lane < n ? ar - r : 1e308;
+    hi = lane < n ? ar : 1e308;
+    tnorm = lane < n ? fabs(ar) + r : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmin(hi, __shfl_xor(hi, o, 64));
+    tnorm = fmax(tnorm, __shfl_xor(tnorm, o, 64));
+  }
+  const double pivmin = fmax(tnorm, 1e-300) * 1e-300 + 1e-300;
+  const double eps = 2.220446049250313e-16;
+  for (int it = 0; it < 12; ++it) {
+    if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const int c = sturm_count_reg(ar, b2r, n, x);
+    const unsigned long long mask = __ballot(c >= 1);
+    if (mask == 0ull) {
+      lo = __shfl(x, 63, 64);
+    } else {
+      const int f = __ffsll((long long)mask) - 1;
+      const double xf = __shfl(x, f, 64);
+      const double xp = __shfl(x, f > 0 ? f - 1 : 0, 64);
+      if (f > 0) lo = xp;
+      hi = xf;
+    }
+  }
+  const double theta = 0.5 * (lo + hi);
+  const double tiny = eps * fmax(tnorm, 1e-300);
+  // Gaussian elimination with partial pivoting of T - theta I (factors once)
+  double u0r = 0.0, u1r = 0.0, u2r = 0.0, fmr = 0.0;
+  int swr = 0;
+  {
+    double c0 = lane_val(ar, 0) - theta, c1 = lane_val(br, 0), c2 = 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+      const double d0 = lane_val(br, i), d1 = lane_val(ar, i + 1) - theta;
+      const double d2 = (i + 1 < n - 1) ? lane_val(br, i + 1) : 0.0;
+      double e0, e1, e2, f;
+      int sw;
+      if (fabs(c0) >= fabs(d0)) {
+        if (c0 == 0.0) c0 = tiny;
+        f = d0 / c0;
+        e0 = c0; e1 = c1; e2 = c2; sw = 0;
+        c0 = d1 - f * c1; c1 = d2 - f * c2; c2 = 0.0;
+      } else {
+        f = c0 / d0;
+        e0 = d0; e1 = d1; e2 = d2; sw = 1;
+        const double n0 = c1 - f * d1, n1 = c2 - f * d2;
+        c0 = n0; c1 = n1; c2 = 0.0;
+      }
+      if (lane == i) { u0r = e0; u1r = e1; u2r = e2; fmr = f; swr = sw; }
+    }
+    if (c0 == 0.0) c0 = tiny;
+    if (lane == n - 1) { u0r = c0; u1r = 0.0; u2r = 0.0; }
+  }
+  const double rinvr = lane < n ? 1.0 / u0r : 0.0;
+  double zr = lane < n ? 1.0 + 0.01 * (double)((lane * 7919) % 13) : 0.0;
+  for (int rep = 0; rep < 2; ++rep) {
+    // forward: apply the row swaps and multipliers to the right-hand side
+    double rc = lane_val(zr, 0);
+    for (int i = 0; i < n - 1; ++i) {
+      const double rd = lane_val(zr, i + 1);
+      const double f = lane_val(fmr, i);
+      double zi;
+      if (lane_int(swr, i)) { zi = rd; rc = rc - f * rd; }
+      else { zi = rc; rc = rd - f * rc; }
+      if (lane == i) zr = zi;
+    }
+    // back substitution, carrying z[i+1] and z[i+2]
+    double z1 = rc * lane_val(rinvr, n - 1);
+    if (lane == n - 1) zr = z1;
+    double z2 = 0.0;
+    if (n >= 2) {
+      const double zn2 = (lane_val(zr, n - 2) - lane_val(u1r, n - 2) * z1) * lane_val(rinvr, n - 2);
+      if (lane == n - 2) zr = zn2;
+      z2 = z1;
+      z1 = zn2;
+    }
+    for (int i = n - 3; i >= 0; --i) {
+      const double zi =
+          (lane_val(zr, i) - lane_val(u1r, i) * z1 - lane_val(u2r, i) * z2) * lane_val(rinvr, i);
+      if (lane == i) zr = zi;
+      z2 = z1;
+      z1 = zi;
+    }
+    const double inv = 1.0 / sqrt(wave_sum(zr * zr));
+    zr *= inv;
+  }
+  // sign: largest |component| positive, the first such index on ties
+  double mx = fabs(zr);
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  const unsigned long long at = __ballot(lane < n && fabs(zr) == mx);
+  const int im = __ffsll((long long)at) - 1;
+  const double sg = lane_val(zr, im) < 0.0 ? -1.0 : 1.0;
+  if (lane == 0) {
+    st[0] = theta;
+    st[1] = bres ? fabs(*bres * lane_val(zr, n - 1)) : 0.0;
+    st[2] = tnorm;
+  }
+  if (lane < n) s[lane] = sg * zr;
+}
+
 __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ ad,
                                                             const double* __restrict__ bd, int n,
                                                             const double* __restrict__ bres,
@@ -462,6 +631,10 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
     const int64_t o = pm.off();
     ad += o; bd += o; s += o; st += o;
     if (bres) bres += o;
+  }
+  if (n >= 2 && n <= 64) {
+    tridiag_lowest_reg(ad, bd, n, bres, s, st);
+    return;
   }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
   __shared__ double u0[kTriMax], u1[kTriMax], u2[kTriMax], z[kTriMax];
