@@ -508,8 +508,10 @@ __device__ int sturm_count_reg(double ar, double b2r, int n, double x) {
 // lane select, so no step waits on LDS.
 __device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* __restrict__ bd,
                                    int n, const double* __restrict__ bres, double* __restrict__ s,
-                                   double* __restrict__ st) {
+                                   double* st, int hinted) {
   const int lane = threadIdx.x;
+  // the previous call's theta (st[0]), read before this call overwrites it
+  const double hint_in = hinted ? st[0] : 0.0;
   const double ar = lane < n ? ad[lane] : 0.0;
   const double br = lane < n - 1 ? bd[lane] : 0.0;
   const double b2r = br * br;
@@ -532,9 +534,31 @@ __device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* 
   }
   const double pivmin = fmax(tnorm, 1e-300) * 1e-300 + 1e-300;
   const double eps = 2.220446049250313e-16;
-  for (int it = 0; it < 12; ++it) {
+  // Multisection. The 64 points of a round may be any increasing sequence
+  // in (lo, hi): the first with a count >= 1 and its predecessor bracket the
+  // lowest eigenvalue whatever the points are. Each round spaces them
+  // evenly, except a first round given a hint h (the lowest Ritz value of
+  // the previous Lanczos step, an upper bound of this one by interlacing,
+  // usually just above it): there they approach h geometrically from below,
+  // h - (h - lo) q^(k+1), so a small move below h is bracketed to within a
+  // fraction of its size in one round. A wrong hint costs one round, never
+  // the result.
+  double h = hint_in;
+  if (!(h > lo && h <= hi)) h = hi;
+  const double dmin = 4.0 * eps * fabs(h) + pivmin;
+  bool geo = hinted && (h - lo) > 4.0 * dmin;
+  for (int it = 0; it < 13; ++it) {
     if (hi - lo <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
-    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    double x;
+    if (geo) {
+      const double d = h - lo;
+      const double lq = log2(dmin / d) / 64.0;  // log2 q, q < 1
+      x = h - d * exp2(lq * (double)(lane + 1));
+      x = fmin(fmax(x, lo), h);
+      geo = false;
+    } else {
+      x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    }
     const int c = sturm_count_reg(ar, b2r, n, x);
     const unsigned long long mask = __ballot(c >= 1);
     if (mask == 0ull) {
@@ -622,18 +646,20 @@ __device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* 
   if (lane < n) s[lane] = sg * zr;
 }
 
+// hinted: st[0] holds the lowest eigenvalue of the previous, one row
+// smaller projection of the same Lanczos run (a search hint only).
 __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ ad,
                                                             const double* __restrict__ bd, int n,
                                                             const double* __restrict__ bres,
                                                             double* __restrict__ s,
-                                                            double* __restrict__ st, PMap pm) {
+                                                            double* st, PMap pm, int hinted) {
   {
     const int64_t o = pm.off();
     ad += o; bd += o; s += o; st += o;
     if (bres) bres += o;
   }
   if (n >= 2 && n <= 64) {
-    tridiag_lowest_reg(ad, bd, n, bres, s, st);
+    tridiag_lowest_reg(ad, bd, n, bres, s, st, hinted);
     return;
   }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
@@ -889,7 +915,7 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
   hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg, kOne);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, kOne);
+                     (const double*)nullptr, z, st, kOne, 0);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec, kOne,
                      (int64_t)0);
@@ -1419,7 +1445,7 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
       hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, J, Vst, tau, dg, eg, kOne);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, J,
-                         (const double*)nullptr, zt, stg, kOne);
+                         (const double*)nullptr, zt, stg, kOne, 0);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, J, zt, k.s, kOne,
                          (int64_t)0);
@@ -1607,7 +1633,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                            k.alpha, k.beta, k.T, kOne);
         TNX_LAUNCH_CHECK();
         hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                           k.beta + j, k.s, k.st, kOne);
+                           k.beta + j, k.s, k.st, kOne, 1);
         TNX_LAUNCH_CHECK();
         // V[j+1] = w / beta[j]
         hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
@@ -1895,7 +1921,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
                        k.alpha, k.beta, k.T, pm);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
-                       k.beta + j, k.s, k.st, pm);
+                       k.beta + j, k.s, k.st, pm, 1);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
                        k.V + (int64_t)(j + 1) * N, N, pm);
@@ -2016,7 +2042,7 @@ int syev_lowest_b(const double* M, int n, int batch, double* evec, double* eval_
   hipLaunchKernelGGL(sytrd_kernel, dim3(1, batch), dim3(256), 0, s, A, n, Vst, tau, dg, eg, pm);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, batch), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, pm);
+                     (const double*)nullptr, z, st, pm, 0);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1, batch), dim3(256), 0, s, Vst, tau, n, z, evec, pm,
                      (int64_t)n);
