@@ -946,67 +946,80 @@ __global__ __launch_bounds__(64) void tridiag_all_kernel(const double* __restric
     const int64_t o = pm.off();
     ad += o; bd += o; w += o; S += o;
   }
-  __shared__ double d[64], e[64];
+  // Implicit QL with eigenvectors (n <= 64). d[i], e[i] live in lane i's
+  // registers: the rotation chain runs on all lanes as wave-uniform scalars,
+  // reading a row with v_readlane and writing lane i's value with a lane
+  // select, so no step waits on LDS. Row k of the eigenvector matrix Z is
+  // private to lane k (in LDS); within a sweep the lane carries the freshly
+  // rotated element Z[k][i] in a register to the next step, so its LDS row
+  // sees only reads of untouched columns and stores nobody waits for.
   __shared__ double Z[64][65];
+  __shared__ double dsh[64];
   const int k = threadIdx.x;
-  if (k < n) {
-    d[k] = ad[k];
-    e[k] = (k < n - 1) ? bd[k] : 0.0;
+  double dr = k < n ? ad[k] : 0.0;
+  double er = k < n - 1 ? bd[k] : 0.0;
+  if (k < n)
     for (int c = 0; c < n; ++c) Z[k][c] = (k == c) ? 1.0 : 0.0;
-  }
-  __syncthreads();
   const double eps = 2.220446049250313e-16;
   for (int l = 0; l < n; ++l) {
     for (int iter = 0; iter < 60; ++iter) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= eps * dd) break;
-      }
+      // m = first index >= l with a negligible e[m] (n - 1 if none)
+      const double dnext = __shfl_down(dr, 1, 64);  // every lane takes part
+      const bool negl = k >= l && k < n - 1 && fabs(er) <= eps * (fabs(dr) + fabs(dnext));
+      const unsigned long long mk = __ballot(negl);
+      const int m = mk ? __ffsll((long long)mk) - 1 : n - 1;
       if (m == l) break;
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      const double el = lane_val(er, l), dl = lane_val(dr, l);
+      double g = (lane_val(dr, l + 1) - dl) / (2.0 * el);
       double r = hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      g = lane_val(dr, m) - dl + el / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
       double sn = 1.0, cs = 1.0, p = 0.0;
-      int i = m - 1;
       bool early = false;
-      for (; i >= l; --i) {
-        double f = sn * e[i];
-        const double b = cs * e[i];
+      double zc = 0.0;  // rotated Z[k][hidx], not yet stored
+      int hidx = -1;    // -1: nothing held
+      for (int i = m - 1; i >= l; --i) {
+        const double ei = lane_val(er, i);
+        double f = sn * ei;
+        const double b = cs * ei;
         // |f|, |g| <= ||T||: no scaling needed for the rotation length
         r = sqrt(fma(f, f, g * g));
-        e[i + 1] = r;
+        if (k == i + 1) er = r;
         if (r == 0.0) {
-          d[i + 1] -= p;
-          e[m] = 0.0;
+          if (k == i + 1) dr -= p;
+          if (k == m) er = 0.0;
           early = true;
           break;
         }
         const double ir = 1.0 / r;
         sn = f * ir;
         cs = g * ir;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * sn + 2.0 * cs * b;
+        g = lane_val(dr, i + 1) - p;
+        r = (lane_val(dr, i) - g) * sn + 2.0 * cs * b;
         p = sn * r;
-        d[i + 1] = g + p;
+        if (k == i + 1) dr = g + p;
         g = cs * r - b;
         if (k < n) {
-          f = Z[k][i + 1];
-          Z[k][i + 1] = sn * Z[k][i] + cs * f;
-          Z[k][i] = cs * Z[k][i] - sn * f;
+          const double zi1 = hidx == i + 1 ? zc : Z[k][i + 1];
+          const double zi = Z[k][i];
+          Z[k][i + 1] = sn * zi + cs * zi1;
+          zc = cs * zi - sn * zi1;
         }
+        hidx = i;
       }
+      // store the held element: Z[k][l] after a full sweep, Z[k][i+1] when
+      // the sweep stopped at step i
+      if (k < n && hidx >= 0) Z[k][hidx] = zc;
       if (early) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[m] = 0.0;
+      if (k == l) { dr -= p; er = g; }
+      if (k == m) er = 0.0;
     }
   }
+  if (k < n) dsh[k] = dr;
   __syncthreads();
   if (k < n) {
-    const double dk = d[k];
+    const double dk = dr;
     int rank = 0;
-    for (int j = 0; j < n; ++j) rank += (d[j] < dk) || (d[j] == dk && j < k);
+    for (int j = 0; j < n; ++j) rank += (dsh[j] < dk) || (dsh[j] == dk && j < k);
     w[rank] = dk;
     for (int row = 0; row < n; ++row) S[row * ldS + rank] = Z[row][k];
   }
