@@ -158,6 +158,23 @@ def test_heff1_matvec(gpu, prec, shape):
     assert relerr(host(y), ref) < 10 * TOL[prec]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_heff1_matvec_split_k(gpu, prec):
+    """chi = 256: the stage-3 contraction (K = chi * D = 1280 over a 256 x 512
+    output) is cut into four K splits with FP64 partial sums, and every
+    operand takes the typed (16-byte buffer load) staging."""
+    shape = (256, 5, 5, 2, 256)
+    print(f"matvec split-K {prec} {shape}", flush=True)
+    rng = np.random.default_rng(256)
+    Lt, W, Rt, x = rand_env(rng, *shape)
+    y = torch.empty(256 * 2 * 256, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_heff1_workspace_bytes(*shape))
+    check(L().tnx_heff1_matvec(dev(Lt).data_ptr(), dev(W).data_ptr(), dev(Rt).data_ptr(), dev(x).data_ptr(),
+                               y.data_ptr(), *shape, PREC[prec], wp, wb, stream()), "matvec")
+    ref = o.heff_matvec(Lt, W, Rt, x)
+    assert relerr(host(y), ref) < 10 * TOL[prec]
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_heff1_dense(gpu, shape):
     chil, Dl, Dr, d, chir = shape
