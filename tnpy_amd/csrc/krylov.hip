@@ -507,12 +507,12 @@ __device__ int sturm_count_reg(double ar, double b2r, int n, double x) {
 // chains that read a row with v_readlane and write lane k's value with a
 // lane select, so no step waits on LDS.
 __device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* __restrict__ bd,
-                                   int n, const double* __restrict__ bres, double* __restrict__ s,
-                                   double* st, int hinted) {
+                                   int n, bool has_bres, double bres_v, bool fin, double a_last,
+                                   double* __restrict__ s, double* st, int hinted) {
   const int lane = threadIdx.x;
   // the previous call's theta (st[0]), read before this call overwrites it
   const double hint_in = hinted ? st[0] : 0.0;
-  const double ar = lane < n ? ad[lane] : 0.0;
+  const double ar = lane < n ? ((fin && lane == n - 1) ? a_last : ad[lane]) : 0.0;
   const double br = lane < n - 1 ? bd[lane] : 0.0;
   const double b2r = br * br;
   // Gershgorin bounds and ||T|| estimate
@@ -640,26 +640,56 @@ __device__ void tridiag_lowest_reg(const double* __restrict__ ad, const double* 
   const double sg = lane_val(zr, im) < 0.0 ? -1.0 : 1.0;
   if (lane == 0) {
     st[0] = theta;
-    st[1] = bres ? fabs(*bres * lane_val(zr, n - 1)) : 0.0;
+    st[1] = has_bres ? fabs(bres_v * lane_val(zr, n - 1)) : 0.0;
     st[2] = tnorm;
   }
   if (lane < n) s[lane] = sg * zr;
 }
 
+// The end of a Lanczos step, folded into the solve that follows it: from
+// the step's CGS sums, alpha[j] = h1[j] + h2[j] and beta[j] = sqrt(h3[j+1])
+// (j = n - 1, the new row; beta[j] is also the residual coupling), written to
+// alpha, beta and the dense T. h1 == nullptr: no step to finish.
+struct TriFinish {
+  const double* h1; const double* h2; const double* h3;
+  double* alpha; double* beta; double* T;
+};
+
 // hinted: st[0] holds the lowest eigenvalue of the previous, one row
 // smaller projection of the same Lanczos run (a search hint only).
+// With a step to finish (f.h1), ad is f.alpha and the new row's alpha and
+// the coupling come from the sums directly (every lane computes them; lane
+// 0 stores them for the later users), never through memory in this kernel.
 __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ ad,
                                                             const double* __restrict__ bd, int n,
                                                             const double* __restrict__ bres,
                                                             double* __restrict__ s,
-                                                            double* st, PMap pm, int hinted) {
-  {
-    const int64_t o = pm.off();
-    ad += o; bd += o; s += o; st += o;
-    if (bres) bres += o;
+                                                            double* st, PMap pm, int hinted,
+                                                            TriFinish f) {
+  const int64_t o = pm.off();
+  ad += o; bd += o; s += o; st += o;
+  if (bres) bres += o;
+  const bool fin = f.h1 != nullptr;
+  double a_last = 0.0, b_new = 0.0;
+  if (fin) {
+    const int j = n - 1;
+    a_last = f.h1[o + j] + f.h2[o + j];
+    b_new = sqrt(fmax(f.h3[o + j + 1], 0.0));
+    if (threadIdx.x == 0) {
+      f.alpha[o + j] = a_last;
+      f.beta[o + j] = b_new;
+      double* T = f.T + o;
+      T[j * kMaxBasis + j] = a_last;
+      if (j + 1 < kMaxBasis) {
+        T[j * kMaxBasis + j + 1] = b_new;
+        T[(j + 1) * kMaxBasis + j] = b_new;
+      }
+    }
   }
+  const bool has_bres = fin || bres != nullptr;
+  const double bres_v = fin ? b_new : (bres ? *bres : 0.0);
   if (n >= 2 && n <= 64) {
-    tridiag_lowest_reg(ad, bd, n, bres, s, st, hinted);
+    tridiag_lowest_reg(ad, bd, n, has_bres, bres_v, fin, a_last, s, st, hinted);
     return;
   }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
@@ -669,7 +699,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   __shared__ double shr[4];
   const int lane = threadIdx.x;
   for (int i = lane; i < n; i += 64) {
-    a[i] = ad[i];
+    a[i] = (fin && i == n - 1) ? a_last : ad[i];
     const double bi = (i < n - 1) ? bd[i] : 0.0;
     b[i] = bi;
     b2[i] = bi * bi;
@@ -679,7 +709,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
     if (lane == 0) {
       s[0] = 1.0;
       st[0] = a[0];
-      st[1] = bres ? fabs(*bres) : 0.0;
+      st[1] = has_bres ? fabs(bres_v) : 0.0;
       st[2] = fabs(a[0]);
     }
     return;
@@ -771,7 +801,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
     for (int i = 1; i < n; ++i) if (fabs(z[i]) > fabs(z[im])) im = i;
     shr[0] = z[im] < 0.0 ? -1.0 : 1.0;
     st[0] = theta;
-    st[1] = bres ? fabs(*bres * z[n - 1]) : 0.0;
+    st[1] = has_bres ? fabs(bres_v * z[n - 1]) : 0.0;
     st[2] = tnorm;
   }
   __syncthreads();
@@ -915,7 +945,7 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
   hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg, kOne);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, kOne, 0);
+                     (const double*)nullptr, z, st, kOne, 0, TriFinish{});
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec, kOne,
                      (int64_t)0);
@@ -1113,27 +1143,6 @@ static void carve(KWork& k, void* ws, int m, size_t N, size_t mvb) {
   k.theta = (double*)take((kMaxBasis + 2) * 8);
   k.st = (double*)take(16 * 8);
   k.scal = (double*)take(16 * 8);
-}
-
-// Finish a Lanczos column j: alpha[j] = h1[j] + h2[j], beta[j] = |w|, and
-// the same entries in the dense projected matrix T.
-__global__ void lanczos_finish_kernel(const double* h1, const double* h2, const double* h3,
-                                      int j, double* alpha, double* beta, double* T, PMap pm) {
-  {
-    const int64_t o = pm.off();
-    h1 += o; h2 += o; h3 += o; alpha += o; beta += o; T += o;
-  }
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double a = h1[j] + h2[j];
-    const double b = sqrt(fmax(h3[j + 1], 0.0));
-    alpha[j] = a;
-    beta[j] = b;
-    T[j * kMaxBasis + j] = a;
-    if (j + 1 < kMaxBasis) {
-      T[j * kMaxBasis + j + 1] = b;
-      T[(j + 1) * kMaxBasis + j] = b;
-    }
-  }
 }
 
 // H y = theta y + (beta_j s_j) V[j+1] for the lowest Ritz pair:
@@ -1458,7 +1467,7 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
       hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, J, Vst, tau, dg, eg, kOne);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, J,
-                         (const double*)nullptr, zt, stg, kOne, 0);
+                         (const double*)nullptr, zt, stg, kOne, 0, TriFinish{});
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, J, zt, k.s, kOne,
                          (int64_t)0);
@@ -1642,11 +1651,9 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         TNX_TRY(cgs_pass(V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
         TNX_TRY(cgs_pass(V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
         TNX_TRY(cgs_pass(V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
-        hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
-                           k.alpha, k.beta, k.T, kOne);
-        TNX_LAUNCH_CHECK();
         hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
-                           k.beta + j, k.s, k.st, kOne, 1);
+                           k.beta + j, k.s, k.st, kOne, 1,
+                           TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
         TNX_LAUNCH_CHECK();
         // V[j+1] = w / beta[j]
         hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
@@ -1930,11 +1937,9 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s, pm, cnt));
     TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s, pm, cnt));
     TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
-    hipLaunchKernelGGL(lanczos_finish_kernel, dim3(1, cnt), dim3(1), 0, s, k.h1, k.h2, k.h3, j,
-                       k.alpha, k.beta, k.T, pm);
-    TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
-                       k.beta + j, k.s, k.st, pm, 1);
+                       k.beta + j, k.s, k.st, pm, 1,
+                       TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
                        k.V + (int64_t)(j + 1) * N, N, pm);
@@ -2055,7 +2060,7 @@ int syev_lowest_b(const double* M, int n, int batch, double* evec, double* eval_
   hipLaunchKernelGGL(sytrd_kernel, dim3(1, batch), dim3(256), 0, s, A, n, Vst, tau, dg, eg, pm);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, batch), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, pm, 0);
+                     (const double*)nullptr, z, st, pm, 0, TriFinish{});
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1, batch), dim3(256), 0, s, Vst, tau, n, z, evec, pm,
                      (int64_t)n);
