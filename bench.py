@@ -65,6 +65,10 @@ def parse():
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--only-batched", action="store_true")
+    p.add_argument("--no-parity", action="store_true",
+                   help="skip the converged mixed-vs-FP64 energy comparison at chi (n=--parity-n)")
+    p.add_argument("--parity-n", type=int, default=24)
+    p.add_argument("--parity-sweeps", type=int, default=6)
     p.add_argument("--batched-serial", action="store_true",
                    help="sweep the chain-length groups one after another instead of concurrently")
     p.add_argument("--two-site", type=int, default=0, metavar="K",
@@ -214,6 +218,53 @@ def batched_leg(args, dev):
         "matvecs": bd.stats["matvecs"],
         "lockstep_matvecs": bd.stats["lockstep_matvecs"],
     }
+
+
+def converged_parity(args, dev):
+    """Energy agreement of the two arithmetics at convergence, at the bench's
+    chi: an XXZ chain of --parity-n sites (n=24 puts its bulk sites at the
+    full chi=1024) swept --parity-sweeps times, alternating direction, from
+    one seeded random MPS in FP64 and in the mixed precision. The same fixed
+    number of sweeps in both, so the comparison does not depend on where a
+    convergence test stops. For delta = 1 the exact ground energy is
+    -(n-1)/4 (the ferromagnetic multiplet; pinned by exact diagonalisation
+    of the oracle Hamiltonian at small n)."""
+    import torch
+    from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
+    from tnpy_amd.model import XXZ
+
+    n, chi = args.parity_n, args.chi
+    mpo = XXZ(n=n, delta=args.delta).mpo
+    mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=args.seed + 1, device=dev, host_rng=False)
+    snap = [t.clone() for t in mps.tensors]
+    del mps
+    out = {
+        "workload": f"XXZ chain n={n} delta={args.delta} chi={chi}, {args.parity_sweeps} one-site sweeps "
+                    f"(tol={args.tol}) from one seeded random MPS in each arithmetic",
+        "max_bond": None,
+    }
+    dirs = [Direction.RIGHTWARD, Direction.LEFTWARD]
+    for prec in ("fp64", args.mixed):
+        d = FiniteDMRG(mpo, bond_dim=chi, mps=MatrixProductState([t.clone() for t in snap], device=dev),
+                       precision=prec, max_basis=args.max_basis, keep=args.keep, compute_variance=False,
+                       device=dev)
+        t0 = time.perf_counter()
+        es = [d.sweep(dirs[k % 2], tol=args.tol) for k in range(args.parity_sweeps)]
+        torch.cuda.synchronize()
+        out[prec] = {"energies": es, "seconds": time.perf_counter() - t0, "matvecs": d.stats["matvecs"]}
+        out["max_bond"] = max(d.mps.bond_dims())
+        log(f"parity {prec}: energies {', '.join(f'{e:.14f}' for e in es)}")
+        del d
+        torch.cuda.empty_cache()
+    e64, emx = out["fp64"]["energies"][-1], out[args.mixed]["energies"][-1]
+    out["abs_dE"] = abs(emx - e64)
+    if args.delta == 1.0:
+        exact = -(n - 1) / 4.0
+        out["exact"] = exact
+        out["fp64_abs_err"] = abs(e64 - exact)
+        out["mixed_abs_err"] = abs(emx - exact)
+    log(f"parity: |dE| mixed vs fp64 after {args.parity_sweeps} sweeps {out['abs_dE']:.3e}")
+    return out
 
 
 def sweep_timed(d, direction, label, step):
@@ -477,6 +528,10 @@ def main():
     if not args.no_batched:
         batched = batched_leg(args, dev)
 
+    parity = None
+    if not args.no_parity:
+        parity = converged_parity(args, dev)
+
     mixed = results[args.mixed]
     r = rates[args.mixed]
     K = args.steps
@@ -514,6 +569,7 @@ def main():
             "matvecs_f64": mixed["matvecs_f64"],
             "matvec_tflops": r["matvec_tflops"],
             "speedup_vs_fp64": mixed["sweeps_per_s"] / results["fp64"]["sweeps_per_s"],
+            "converged": parity,
         },
         "roofline": {
             "bound": "mfma",
