@@ -1299,6 +1299,40 @@ struct EigCtx {
   double* gsm = nullptr;  // its small arrays (8 x (kMaxBasis + 8))
 };
 
+// Pinned host buffer for the per-step status of the single-problem Lanczos
+// (one per host thread, kept for the life of the process): a copy into it
+// can be waited for with an event while later work stays queued behind it.
+// Null when the allocation fails (the solver then reads its status the
+// plain way).
+static double* pinned_status() {
+  thread_local double* p = nullptr;
+  thread_local bool tried = false;
+  if (!tried) {
+    tried = true;
+    if (hipHostMalloc((void**)&p, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      hipGetLastError();
+    }
+  }
+  return p;
+}
+
+// An event for the duration of one solve.
+struct StepEvent {
+  hipEvent_t e = nullptr;
+  StepEvent() {
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+      hipGetLastError();
+    }
+  }
+  ~StepEvent() {
+    if (e) hipEventDestroy(e);
+  }
+  StepEvent(const StepEvent&) = delete;
+  StepEvent& operator=(const StepEvent&) = delete;
+};
+
 static int apply_h(EigCtx& c, const void* x, int x_f32, double* y, int prec) {
   c.matvecs++;
   if (prec == TNX_F64) c.matvecs64++;
@@ -1582,6 +1616,14 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
   int cprec = prec;
   double rho_prev = 0.0;
   bool first_cycle = true;
+  // Step pipelining: the next step's matvec and CGS passes (they need only
+  // V[j+1]) are queued before the host reads step j's status, so the GPU
+  // works while the host decides. When step j ends the cycle, that queued
+  // step is discarded (its buffers are rewritten before any later use) and
+  // not counted.
+  StepEvent sev;
+  double* pin = pinned_status();
+  const bool pipelined = sev.e != nullptr && pin != nullptr;
 
   while (true) {
     TNX_TRY(outer_step(c, &theta, &rho));
@@ -1635,22 +1677,31 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + 1, V + N, N,
                          kOne);
       TNX_LAUNCH_CHECK();
-      int j = 1;
-      while (true) {
+      // Step jj's matvec and CGS passes: w = H V[jj] orthogonalised against
+      // V[0..jj] (sums in h1, h2, h3).
+      auto front = [&](int jj) -> int {
         if constexpr (f32) {
           // the matvec reads FP64 input faster than FP32 (16-byte vs 8-byte
-          // tile loads): widen V[j] into k.y, which is free until the cycle's
-          // Ritz vector is formed
-          hipLaunchKernelGGL(to_f64_kernel, dim3(nb), dim3(256), 0, s, V + (int64_t)j * N, k.y, N);
+          // tile loads): widen V[jj] into k.y, which is free until the
+          // cycle's Ritz vector is formed
+          hipLaunchKernelGGL(to_f64_kernel, dim3(nb), dim3(256), 0, s, V + (int64_t)jj * N, k.y, N);
           TNX_LAUNCH_CHECK();
           TNX_TRY(apply_h(c, k.y, 0, k.w, cprec));
         } else {
-          TNX_TRY(apply_h(c, V + (int64_t)j * N, 0, k.w, cprec));
+          TNX_TRY(apply_h(c, V + (int64_t)jj * N, 0, k.w, cprec));
         }
+        const int JJ = jj + 1;
+        TNX_TRY(cgs_pass(V, N, JJ, k.w, nullptr, true, k.partials, k.h1, N, s));
+        TNX_TRY(cgs_pass(V, N, JJ, k.w, k.h1, true, k.partials, k.h2, N, s));
+        TNX_TRY(cgs_pass(V, N, JJ, k.w, k.h2, false, k.partials, k.h3, N, s));
+        return TNX_OK;
+      };
+      int j = 1;
+      bool queued = false;  // step j's front already launched
+      while (true) {
         const int J = j + 1;
-        TNX_TRY(cgs_pass(V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s));
-        TNX_TRY(cgs_pass(V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s));
-        TNX_TRY(cgs_pass(V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s));
+        if (!queued) TNX_TRY(front(j));
+        const int mv_now = c.matvecs, mv64_now = c.matvecs64;
         hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
                            k.beta + j, k.s, k.st, kOne, 1,
                            TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
@@ -1660,7 +1711,18 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                            V + (int64_t)(j + 1) * N, N, kOne);
         TNX_LAUNCH_CHECK();
         double h[4];
-        TNX_TRY(read_status(c, h, 3));
+        // the next step goes ahead only where this one, not ending the
+        // cycle, would be followed by it (no restart, not exhausted)
+        const bool ahead = pipelined && J < mb && J < N && mv_now < max_mv;
+        if (ahead) {
+          TNX_HIP(hipMemcpyAsync(pin, k.st, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+          TNX_HIP(hipEventRecord(sev.e, s));
+          TNX_TRY(front(j + 1));
+          TNX_HIP(hipEventSynchronize(sev.e));
+          for (int q = 0; q < 3; ++q) h[q] = pin[q];
+        } else {
+          TNX_TRY(read_status(c, h, 3));
+        }
         const double theta_in = h[0];
         const double est = h[1];
         anorm = std::max(anorm, h[2]);
@@ -1668,7 +1730,11 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         // FP64 cycles test against the current ||H|| estimate (it grows with
         // the Ritz values seen); low-precision cycles against their fixed target
         const double itol = mixed ? inner_tol : tol * anorm;
-        if (est <= itol || exhausted || c.matvecs >= max_mv) {
+        if (est <= itol || exhausted || mv_now >= max_mv) {
+          if (ahead) {  // the queued step is not used
+            c.matvecs = mv_now;
+            c.matvecs64 = mv64_now;
+          }
           last_J = J;
           // Ritz vector y = V[0..j] s
           hipLaunchKernelGGL(vcomb_kernel<VT>, dim3(nb), dim3(256), 0, s, V, N, J, k.s, k.y, N,
@@ -1728,8 +1794,10 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                                  hipMemcpyDeviceToDevice, s));
           restarts++;
           j = kp;
+          queued = false;
         } else {
           j++;
+          queued = ahead;
         }
       }
     };
