@@ -1317,6 +1317,26 @@ static double* pinned_status() {
   return p;
 }
 
+// Pinned host buffer of at least n doubles (one per host thread, grown on
+// demand, kept for the life of the process); null when allocation fails.
+static double* pinned_buffer(size_t n) {
+  thread_local double* p = nullptr;
+  thread_local size_t cap = 0;
+  if (n > cap) {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n, 1024);
+    if (hipHostMalloc((void**)&p, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      hipGetLastError();
+      return nullptr;
+    }
+    cap = want;
+  }
+  return p;
+}
+
 // An event for the duration of one solve.
 struct StepEvent {
   hipEvent_t e = nullptr;
@@ -1996,15 +2016,28 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
                        k.h3 + 1, k.V + N, N, pm);
     TNX_LAUNCH_CHECK();
   }
+  // Step pipelining as in the single-problem solver: the next step's
+  // matvec and CGS passes for every active problem are queued before the
+  // host reads this step's statuses; problems that finish here drop theirs
+  // (not counted), the others take it as their next step.
+  StepEvent sev;
+  double* pin = pinned_buffer((size_t)3 * batch);
+  const bool pipelined = sev.e != nullptr && pin != nullptr;
+  auto front = [&](int jj) -> int {
+    TNX_TRY(matvec(k.V + (int64_t)jj * N, k.w, cnt));
+    const int JJ = jj + 1;
+    TNX_TRY(cgs_pass(k.V, N, JJ, k.w, nullptr, true, k.partials, k.h1, N, s, pm, cnt));
+    TNX_TRY(cgs_pass(k.V, N, JJ, k.w, k.h1, true, k.partials, k.h2, N, s, pm, cnt));
+    TNX_TRY(cgs_pass(k.V, N, JJ, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
+    return TNX_OK;
+  };
   int j = 1;
+  bool queued = false;  // step j's front already launched for the active set
   while (cnt > 0) {
     const int nb = vec_blocks(N, cnt);
-    double* vj = k.V + (int64_t)j * N;
-    TNX_TRY(matvec(vj, k.w, cnt));
     const int J = j + 1;
-    TNX_TRY(cgs_pass(k.V, N, J, k.w, nullptr, true, k.partials, k.h1, N, s, pm, cnt));
-    TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h1, true, k.partials, k.h2, N, s, pm, cnt));
-    TNX_TRY(cgs_pass(k.V, N, J, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
+    if (!queued) TNX_TRY(front(j));
+    const int mv_now = matvecs;
     hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
                        k.beta + j, k.s, k.st, pm, 1,
                        TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
@@ -2012,9 +2045,20 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
                        k.V + (int64_t)(j + 1) * N, N, pm);
     TNX_LAUNCH_CHECK();
-    TNX_TRY(status(0, 3, cnt));
+    const bool ahead = pipelined && J < mb && J < N && mv_now < max_mv;
+    if (ahead) {
+      hipLaunchKernelGGL(gather_status_kernel, dim3(1, cnt), dim3(64), 0, s, k.st, 0, 3, stat, pm);
+      TNX_LAUNCH_CHECK();
+      TNX_HIP(hipMemcpyAsync(pin, stat, sizeof(double) * 3 * cnt, hipMemcpyDeviceToHost, s));
+      TNX_HIP(hipEventRecord(sev.e, s));
+      TNX_TRY(front(j + 1));
+      TNX_HIP(hipEventSynchronize(sev.e));
+      std::copy(pin, pin + (size_t)3 * cnt, hs.begin());
+    } else {
+      TNX_TRY(status(0, 3, cnt));
+    }
     const bool exhausted = (J >= N);
-    const bool limit = matvecs >= max_mv;
+    const bool limit = mv_now >= max_mv;
     std::vector<int> rest;
     fin.clear();
     for (int i = 0; i < cnt; ++i) {
@@ -2026,7 +2070,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
         fin.push_back(b);
         info[b].eval = th;
         info[b].resnorm = est;
-        info[b].matvecs = info[b].matvecs_f64 = matvecs;
+        info[b].matvecs = info[b].matvecs_f64 = mv_now;
         info[b].restarts = restarts;
         info[b].converged = conv ? 1 : 0;
         any_fail |= !conv;
@@ -2080,8 +2124,10 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
       TNX_LAUNCH_CHECK();
       restarts++;
       j = keep;
+      queued = false;
     } else {
       j++;
+      queued = ahead;
     }
   }
   TNX_HIP(hipStreamSynchronize(s));
