@@ -2,28 +2,39 @@
 
 Workload (the chi=1024 configuration of the headline metric): XXZ chain,
 n=200, delta=1.0 (tnpy's sign convention: the ferromagnetic Heisenberg
-chain), MPO bond D=5, d=2, one-site DMRG with chi=1024 from a seeded random
-MPS, eigen tolerance 1e-8. A "step" is one directional sweep (n-1 site
-updates: Krylov eigensolve, perturbation, QR gauge move, environment
-update), as ``tnpy/finite_dmrg.py:143-171``.
+chain), MPO bond D=5, d=2, one-site DMRG at chi=1024, eigen tolerance 1e-8.
+A "step" is one directional sweep (n-1 site updates: Krylov eigensolve,
+perturbation, QR gauge move, environment update), as
+``tnpy/finite_dmrg.py:143-171``.
 
-Run: W untimed warm-up sweeps in FP64 from the seeded state, then snapshot
-the state and time exactly K sweeps in FP64 and K sweeps in the mixed
-precision, each from the same snapshot. The first site of the timed sweep is the identical local
-eigenproblem in both precisions, so its |dE| measures the mixed solver's
-accuracy at chi=1024; the end-of-sweep |dE| also reflects the two
-trajectories (the chain is far from converged after one sweep from a random
-state, and its ground space is degenerate). ``value`` is the mixed path's
-sweeps/s (FP32 Krylov matvecs with FP64 residual correction by default;
-BF16x3's ~1e-5 operator error cannot resolve this chain's clustered local
-spectra, so it escalates to FP64 and is slower here); the FP64 figure and
-matvec TFLOP/s of every precision are reported beside it. The dominant
-kernel's roofline comes from HIP-event timing of the matvec GEMM at its real
-shape on the library's stream.
+The whole run keeps to a wall-clock budget (``--budget`` seconds after the
+library is loaded), so the number of timed sweeps can be below ``--steps``;
+``steps`` in the output line is the number actually timed in each
+precision. The order:
 
-The CPU baseline times the numpy FP64 restatement of the same site update
-(test oracle, scipy/OpenBLAS on the host cores) on a bounded sample and
-scales it to a sweep.
+1. Kernel rates at chi (HIP events on the library's stream): the H_eff
+   matvec and its stage-1 GEMM in FP64 and the mixed arithmetic (the
+   ``roofline`` object), and the QR gauge move.
+2. Warm state, untimed: one-site DMRG at ``--warm-chi`` from a seeded
+   random MPS until the energy changes by less than tol (a bond-dimension
+   ramp), then the bonds are widened to chi (zero padding, Householder LQ
+   sweep: every chi=1024 tensor is a full orthonormal basis) and ``--warmup``
+   FP64 sweeps run at chi. The reference's own sweep clock is the mean of
+   the fastest sweeps of a run (``tnpy/finite_dmrg.py:252-256``): the
+   converged end of a run, which this reproduces.
+3. Timed: the same number of sweeps in FP64 and in the mixed precision, each
+   from the same snapshot of the warm state (``value`` = mixed sweeps/s).
+4. Krylov sample: the convergence phase of a run (the first sweep from a
+   random state), where the eigensolver dominates: ``--krylov-sites``
+   updates at the middle of the chain from a seeded random chi=1024 MPS, in
+   FP64 and in the mixed precision from the same state. Its first site is
+   the identical local eigenproblem in both, so its |dE| measures the mixed
+   solver's accuracy at chi=1024.
+5. CPU baseline: the numpy FP64 restatement of the same site update (test
+   oracle, scipy/OpenBLAS on the host cores) on a bounded sample, scaled to
+   a sweep with the GPU's matvec count per site.
+6. Many-chain leg (config of 256 random-coupling chains at chi=64), within
+   what is left of the budget.
 """
 from __future__ import annotations
 
@@ -38,17 +49,20 @@ import numpy as np
 METRIC = "DMRG sweeps/s & matvec TFLOP/s @χ=1024: mixed vs FP64, |ΔE|<1e-10"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2516.6 / 3.0}
 HBM_GBS = 8000.0
+_T0 = time.perf_counter()
 
 
 def log(msg: str) -> None:
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=1)
-    p.add_argument("--warmup", type=int, default=0)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--budget", type=float, default=300.0,
+                   help="wall-clock seconds for the whole run after the library is loaded")
     p.add_argument("--n", type=int, default=200)
     p.add_argument("--chi", type=int, default=1024)
     p.add_argument("--delta", type=float, default=1.0)
@@ -57,6 +71,11 @@ def parse():
     p.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     p.add_argument("--max-basis", type=int, default=64)
     p.add_argument("--keep", type=int, default=20)
+    p.add_argument("--warm-chi", type=int, default=16,
+                   help="bond dimension of the ramp that prepares the warm state")
+    p.add_argument("--warm-sweeps", type=int, default=30, help="sweep limit of the ramp")
+    p.add_argument("--krylov-sites", type=int, default=6,
+                   help="site updates of the Krylov sample (0 skips it)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=2)
     p.add_argument("--no-batched", action="store_true")
@@ -64,13 +83,16 @@ def parse():
     p.add_argument("--batch-chi", type=int, default=64)
     p.add_argument("--batch-lengths", default="32,48,64,96")
     p.add_argument("--batch-seed", type=int, default=7)
+    p.add_argument("--batch-sweeps", type=int, default=2)
     p.add_argument("--only-batched", action="store_true")
-    p.add_argument("--no-parity", action="store_true",
-                   help="skip the converged mixed-vs-FP64 energy comparison at chi (n=--parity-n)")
+    p.add_argument("--batched-concurrent", action="store_true",
+                   help="sweep the chain-length groups concurrently (one host thread and stream each) "
+                        "instead of one after another")
+    p.add_argument("--parity", action="store_true",
+                   help="also run the converged mixed-vs-FP64 comparison from a random state at chi "
+                        "(n=--parity-n, --parity-sweeps sweeps each)")
     p.add_argument("--parity-n", type=int, default=24)
     p.add_argument("--parity-sweeps", type=int, default=6)
-    p.add_argument("--batched-serial", action="store_true",
-                   help="sweep the chain-length groups one after another instead of concurrently")
     p.add_argument("--two-site", type=int, default=0, metavar="K",
                    help="only time K two-site bond updates at the middle of a random-field Heisenberg "
                         "chain (n=--two-site-n, h=0.5, chi=--chi) in FP64 and in the mixed precision")
@@ -85,6 +107,16 @@ def parse():
     return p.parse_args()
 
 
+class Budget:
+    """Wall-clock budget shared by the legs of one run."""
+
+    def __init__(self, seconds: float):
+        self.end = time.perf_counter() + seconds
+
+    def left(self) -> float:
+        return self.end - time.perf_counter()
+
+
 def copy_reference(mib=256, reps=10):
     """A plain device-to-device copy of known size (torch's copy kernel):
     reads and writes mib MiB per launch."""
@@ -95,6 +127,241 @@ def copy_reference(mib=256, reps=10):
     ms = event_time(lambda: y.copy_(x), reps)
     return {"bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8, "ms": ms,
             "GBps": 2 * x.numel() * 8 / ms / 1e6}
+
+
+def event_time(fn, reps):
+    """Average ms per call of fn over reps calls, HIP events on torch's
+    current stream (the stream every libtnx call here is issued on)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def kernel_rates(chi, D, d, precisions):
+    """Matvec, dominant-GEMM and gauge-move rates at (chi, D, d)."""
+    import torch
+
+    from tnpy_amd import _lib
+
+    L = _lib.lib()
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    Lt = torch.randn(chi, D, chi, dtype=torch.float64, device=dev, generator=g)
+    Rt = torch.randn(chi, D, chi, dtype=torch.float64, device=dev, generator=g)
+    W = torch.randn(D, D, d, d, dtype=torch.float64, device=dev, generator=g)
+    x = torch.randn(chi, d, chi, dtype=torch.float64, device=dev, generator=g)
+    y = torch.empty_like(x)
+    ws = torch.empty(L.tnx_heff1_workspace_bytes(chi, D, D, d, chi), dtype=torch.uint8, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    F1 = 4 * D * d * chi**3 + 2 * D * D * d * d * chi * chi
+    A = torch.randn(chi * d, chi, dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn(chi, D * chi, dtype=torch.float64, device=dev, generator=g)
+    Cm = torch.empty(chi * d, D * chi, dtype=torch.float64, device=dev)
+    FG = 2 * (chi * d) * (D * chi) * chi
+    out = {}
+    for p in precisions:
+        pi = _lib.PRECISIONS[p]
+        tm = event_time(
+            lambda: L.tnx_heff1_matvec(Lt.data_ptr(), W.data_ptr(), Rt.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                       chi, D, D, d, chi, pi, ws.data_ptr(), ws.numel(), s), 10)
+        tg = event_time(
+            lambda: L.tnx_gemm_f64(chi * d, D * chi, chi, 1, 1.0, A.data_ptr(), chi, 0, 1, B.data_ptr(), D * chi,
+                                   0, 1, 0.0, Cm.data_ptr(), D * chi, 0, pi, s), 20)
+        out[p] = {
+            "matvec_ms": tm,
+            "matvec_tflops": F1 / tm / 1e9,
+            "gemm_ms": tg,
+            "gemm_tflops": FG / tg / 1e9,
+            "gemm_flops_per_launch": FG,
+        }
+    # gauge move (QR of the chi*d x chi site matrix, R times the next site)
+    site = torch.randn(chi, d, chi, dtype=torch.float64, device=dev, generator=g)
+    work = site.clone()
+    nxt = torch.randn(chi, d, chi, dtype=torch.float64, device=dev, generator=g)
+    nout = torch.empty_like(nxt)
+    gws = torch.empty(L.tnx_gauge_workspace_bytes(chi, d, chi), dtype=torch.uint8, device=dev)
+
+    def gauge():
+        work.copy_(site)
+        L.tnx_gauge_right(work.data_ptr(), nxt.data_ptr(), nout.data_ptr(), chi, d, chi, d * chi,
+                          gws.data_ptr(), gws.numel(), s)
+
+    tq = event_time(gauge, 5)
+    m, n = chi * d, chi
+    fq = 2 * (2 * m * n * n - 2 * n**3 / 3) + 2 * n * n * (d * chi)  # QR + thin Q + R*next
+    out["gauge"] = {"ms": tq, "tflops": fq / tq / 1e9, "flops": fq}
+    return out
+
+
+def measured_traffic(precision, M, N, K):
+    """HBM bytes per launch of the stage-1 GEMM from the committed counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by a plain
+    copy's reading), or None when no summary matches this shape."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_stage1_chi1024_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except OSError:
+        return None
+    for t in data.get("entries", [data]):
+        if (t.get("precision"), t.get("M"), t.get("N"), t.get("K")) == (precision, M, N, K):
+            return t["bytes_per_launch"]
+    return None
+
+
+def make_dmrg(args, mpo, state, prec, dev, chi):
+    from tnpy_amd import FiniteDMRG
+
+    return FiniteDMRG(mpo, bond_dim=chi, mps=state, precision=prec, max_basis=args.max_basis,
+                      keep=args.keep, compute_variance=False, device=dev)
+
+
+def timed_sweep(d, direction, tol, label):
+    """One sweep; returns (seconds, energy, matvecs, fp64 matvecs)."""
+    import torch
+
+    n = d.n_sites
+    t0 = time.perf_counter()
+
+    def progress(site, energy):
+        done = site + 1 if direction.value == 1 else n - site
+        if done % 50 == 0:
+            log(f"  {label}: {done}/{n - 1} sites, E={energy:.12f}, {time.perf_counter() - t0:.1f}s")
+
+    d.on_site = progress
+    mv0, mv640 = d.stats["matvecs"], d.stats["matvecs_f64"]
+    e = d.sweep(direction, tol=tol)
+    torch.cuda.synchronize()
+    d.on_site = None
+    return time.perf_counter() - t0, e, d.stats["matvecs"] - mv0, d.stats["matvecs_f64"] - mv640
+
+
+def warm_state(args, mpo, dev, budget):
+    """Bond-dimension ramp: DMRG at --warm-chi from the seeded random MPS
+    until converged, then the bonds widened to chi."""
+    import torch
+
+    from tnpy_amd import Direction, MatrixProductState
+
+    n, chi = args.n, args.chi
+    t = time.perf_counter()
+    mps = MatrixProductState.random(n=n, bond_dim=args.warm_chi, phys_dim=2, seed=args.seed, device=dev)
+    d = make_dmrg(args, mpo, mps, "fp64", dev, args.warm_chi)
+    dirs = [Direction.RIGHTWARD, Direction.LEFTWARD]
+    energies = []
+    for k in range(args.warm_sweeps):
+        e = d.sweep(dirs[k % 2], tol=args.tol)
+        energies.append(e)
+        if len(energies) >= 2 and abs(energies[-1] - energies[-2]) < args.tol and k % 2 == 1:
+            break
+        if budget.left() < 0.8 * budget.total:
+            break
+    torch.cuda.synchronize()
+    ramp = {"chi": args.warm_chi, "sweeps": len(energies), "energy": energies[-1],
+            "seconds": time.perf_counter() - t}
+    log(f"ramp at chi={args.warm_chi}: {len(energies)} sweeps, E={energies[-1]:.14f}, {ramp['seconds']:.1f}s")
+    # the ramp ends with a leftward sweep: centre at site 0, ready for a rightward sweep
+    big = d.mps.grow(chi)
+    del d
+    torch.cuda.synchronize()
+    log(f"state widened to bonds max {big.bond_dim}")
+    return big, ramp
+
+
+def krylov_sample(args, mpo, dev, budget):
+    """The convergence phase: --krylov-sites rightward updates at the chain
+    middle from a seeded random chi MPS (centre moved there by QR gauge
+    moves, which leave the state unchanged), FP64 and the mixed precision
+    from the same state."""
+    import torch
+
+    from tnpy_amd import Direction, MatrixProductState, _lib
+
+    n, chi, K = args.n, args.chi, args.krylov_sites
+    mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=args.seed + 1, device=dev, host_rng=False)
+    mid = max(1, n // 2 - K // 2)
+    ws = _lib.Workspace(dev)
+    for site in range(mid):
+        mps.split_tensor(site, Direction.RIGHTWARD, workspace=ws)
+    snap = [t.clone() for t in mps.tensors]
+    del mps, ws
+    torch.cuda.synchronize()
+    out = {}
+    for prec in ("fp64", args.mixed):
+        d = make_dmrg(args, mpo, MatrixProductState([t.clone() for t in snap], device=dev), prec, dev, chi)
+        d.environment.ensure_left(mid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        energies = []
+        for site in range(mid, min(mid + K, n - 1)):
+            energies.append(d._site_step(site, Direction.RIGHTWARD, args.tol))
+            if budget.left() < 0:
+                break
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[prec] = {"seconds": dt, "sites": len(energies), "s_per_site": dt / len(energies),
+                     "matvecs": d.stats["matvecs"], "matvecs_f64": d.stats["matvecs_f64"], "energies": energies}
+        log(f"krylov sample {prec}: {len(energies)} sites in {dt:.2f}s, matvecs {d.stats['matvecs']} "
+            f"(f64 {d.stats['matvecs_f64']}), E={energies[-1]:.12f}")
+        del d
+        torch.cuda.empty_cache()
+    e64, emx = out["fp64"]["energies"], out[args.mixed]["energies"]
+    k = min(len(e64), len(emx))
+    return {
+        "workload": (f"XXZ n={n} delta={args.delta} chi={chi}: {k} one-site updates at sites {mid}..{mid + k - 1} "
+                     f"of the first rightward sweep from a seeded random MPS, tol={args.tol}"),
+        "fp64": {q: v for q, v in out["fp64"].items() if q != "energies"},
+        "mixed": {q: v for q, v in out[args.mixed].items() if q != "energies"},
+        "abs_dE_first_site": abs(e64[0] - emx[0]),
+        "rel_dE_first_site": abs(e64[0] - emx[0]) / abs(e64[0]),
+        "speedup_vs_fp64": (out["fp64"]["s_per_site"] / out[args.mixed]["s_per_site"]),
+    }
+
+
+def cpu_baseline(chi, D, d, n, mv_per_site, bond, reps):
+    """Bounded CPU sample: the oracle's matvec, SVD gauge move and
+    environment update at chi x chi, each timed ``reps`` times on the host
+    cores, scaled to one sweep of the same chain."""
+    from oracle import dmrg_ref as o
+
+    rng = np.random.default_rng(0)
+    Lt = rng.standard_normal((chi, D, chi))
+    Rt = rng.standard_normal((chi, D, chi))
+    W = o.xxz_bulk(1.0)
+    x = rng.standard_normal((chi, d, chi))
+
+    def best(fn):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts)
+
+    t_mv = best(lambda: o.heff_matvec(Lt, W, Rt, x))
+    arr = [x.copy(), rng.standard_normal((chi, d, chi))]
+    t_svd = best(lambda: o.split_tensor_svd([a.copy() for a in arr], 0, o.RIGHTWARD))
+    t_env = best(lambda: o.env_left(Lt, x, W))
+    # scale per site by the bond dimensions of the chain (cost ~ chil*chir*(chil+chir))
+    dims = [1] + bond + [1]
+    ref = 2.0 * chi**3
+    total = 0.0
+    for i in range(n - 1):
+        l, r = dims[i], dims[i + 1]
+        f = l * r * (l + r) / ref
+        total += f * ((mv_per_site + 1.0) * t_mv + t_env) + (l * d * r * r) / (d * chi**3) * t_svd
+    return {"sweep_s": total, "t_matvec_s": t_mv, "t_svd_s": t_svd, "t_env_s": t_env}
 
 
 def two_site_leg(args, dev):
@@ -151,15 +418,14 @@ def two_site_leg(args, dev):
     }
 
 
-def batched_leg(args, dev):
+def batched_leg(args, dev, budget=None):
     """Many-chain workload: random-coupling Heisenberg chains (J_i ~ U(0,1)
     per bond from a per-chain seed), lengths drawn uniformly from
     --batch-lengths, chi=--batch-chi, FP64, all swept together by
-    BatchedFiniteDMRG (one launch per kernel per site step and length
-    group). The same number of sweeps is then run chain by chain with
-    FiniteDMRG on one sample chain per length, from the same initial
-    states: their energies must agree, and their times scaled by the
-    number of chains of each length give the unbatched FP64 time."""
+    BatchedFiniteDMRG. The same number of sweeps is then run chain by chain
+    with FiniteDMRG on one sample chain per length, from the same initial
+    states: their energies must agree, and their times scaled by the number
+    of chains of each length give the unbatched FP64 time."""
     import torch
 
     from tnpy_amd import Direction, FiniteDMRG
@@ -173,7 +439,7 @@ def batched_leg(args, dev):
     t = time.perf_counter()
     bd = BatchedFiniteDMRG([m.mpo for m in models], chi=chi,
                            seeds=[args.batch_seed * 100003 + i for i in range(len(ns))],
-                           max_basis=args.max_basis, keep=args.keep, concurrent=not args.batched_serial,
+                           max_basis=args.max_basis, keep=args.keep, concurrent=args.batched_concurrent,
                            device=dev)
     torch.cuda.synchronize()
     log(f"batched: {len(ns)} chains {dict(sorted((n, ns.count(n)) for n in set(ns)))} chi={chi} "
@@ -184,10 +450,14 @@ def batched_leg(args, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eb = []
-    for k in range(args.steps):
+    sweeps = max(1, args.batch_sweeps)
+    for k in range(sweeps):
         eb.append(bd.sweep(directions[k % 2], tol=args.tol))
         torch.cuda.synchronize()
         log(f"batched sweep {k + 1}: {time.perf_counter() - t0:.2f}s")
+        if budget is not None and budget.left() < 3.0 * (time.perf_counter() - t0) / (k + 1):
+            sweeps = k + 1
+            break
     dt = time.perf_counter() - t0
     single = {}
     dEmax = 0.0
@@ -196,22 +466,24 @@ def batched_leg(args, dev):
                        compute_variance=False, device=dev)
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        es = [f.sweep(directions[k % 2], tol=args.tol) for k in range(args.steps)]
+        es = [f.sweep(directions[k % 2], tol=args.tol) for k in range(sweeps)]
         torch.cuda.synchronize()
         single[ns[i]] = time.perf_counter() - ts
         dEmax = max(dEmax, abs(es[-1] - eb[-1][i]) / abs(es[-1]))
     est = sum(single[n] for n in ns)
-    log(f"batched: {args.steps} sweep(s) of {len(ns)} chains in {dt:.2f}s; unbatched estimate {est:.1f}s "
+    log(f"batched: {sweeps} sweep(s) of {len(ns)} chains in {dt:.2f}s; unbatched estimate {est:.1f}s "
         f"(per-length single-chain sweep times {single}); max rel |dE| batched vs single {dEmax:.2e}")
     return {
         "workload": (f"{len(ns)} random-coupling Heisenberg chains, J_i~U(0,1), n in {lengths}, chi={chi}, "
-                     f"one-site DMRG sweep from seeded random states, FP64, tol={args.tol}"),
+                     f"one-site DMRG sweeps from seeded random states, FP64, tol={args.tol}"),
         "chains": len(ns),
         "lengths": {str(n): ns.count(n) for n in sorted(set(ns))},
+        "sweeps": sweeps,
+        "concurrent_groups": args.batched_concurrent,
         "seconds": dt,
-        "sweeps_per_s": args.steps / dt,
-        "chain_sweeps_per_s": len(ns) * args.steps / dt,
-        "ms_per_step": 1e3 * dt / args.steps,
+        "sweeps_per_s": sweeps / dt,
+        "chain_sweeps_per_s": len(ns) * sweeps / dt,
+        "ms_per_step": 1e3 * dt / sweeps,
         "unbatched_fp64_est_seconds": est,
         "speedup_vs_unbatched_fp64": est / dt,
         "sample_rel_dE_vs_unbatched": dEmax,
@@ -221,16 +493,13 @@ def batched_leg(args, dev):
 
 
 def converged_parity(args, dev):
-    """Energy agreement of the two arithmetics at convergence, at the bench's
-    chi: an XXZ chain of --parity-n sites (n=24 puts its bulk sites at the
+    """Energy agreement of the two arithmetics at convergence from a random
+    state: an XXZ chain of --parity-n sites (n=24 puts its bulk sites at the
     full chi=1024) swept --parity-sweeps times, alternating direction, from
-    one seeded random MPS in FP64 and in the mixed precision. The same fixed
-    number of sweeps in both, so the comparison does not depend on where a
-    convergence test stops. For delta = 1 the exact ground energy is
-    -(n-1)/4 (the ferromagnetic multiplet; pinned by exact diagonalisation
-    of the oracle Hamiltonian at small n)."""
+    one seeded random MPS in FP64 and in the mixed precision."""
     import torch
-    from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
+
+    from tnpy_amd import Direction, MatrixProductState
     from tnpy_amd.model import XXZ
 
     n, chi = args.parity_n, args.chi
@@ -238,170 +507,22 @@ def converged_parity(args, dev):
     mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=args.seed + 1, device=dev, host_rng=False)
     snap = [t.clone() for t in mps.tensors]
     del mps
-    out = {
-        "workload": f"XXZ chain n={n} delta={args.delta} chi={chi}, {args.parity_sweeps} one-site sweeps "
-                    f"(tol={args.tol}) from one seeded random MPS in each arithmetic",
-        "max_bond": None,
-    }
+    out = {"workload": f"XXZ chain n={n} delta={args.delta} chi={chi}, {args.parity_sweeps} one-site sweeps "
+                       f"(tol={args.tol}) from one seeded random MPS in each arithmetic"}
     dirs = [Direction.RIGHTWARD, Direction.LEFTWARD]
     for prec in ("fp64", args.mixed):
-        d = FiniteDMRG(mpo, bond_dim=chi, mps=MatrixProductState([t.clone() for t in snap], device=dev),
-                       precision=prec, max_basis=args.max_basis, keep=args.keep, compute_variance=False,
-                       device=dev)
+        d = make_dmrg(args, mpo, MatrixProductState([t.clone() for t in snap], device=dev), prec, dev, chi)
         t0 = time.perf_counter()
         es = [d.sweep(dirs[k % 2], tol=args.tol) for k in range(args.parity_sweeps)]
         torch.cuda.synchronize()
         out[prec] = {"energies": es, "seconds": time.perf_counter() - t0, "matvecs": d.stats["matvecs"]}
-        out["max_bond"] = max(d.mps.bond_dims())
         log(f"parity {prec}: energies {', '.join(f'{e:.14f}' for e in es)}")
         del d
         torch.cuda.empty_cache()
-    e64, emx = out["fp64"]["energies"][-1], out[args.mixed]["energies"][-1]
-    out["abs_dE"] = abs(emx - e64)
+    out["abs_dE"] = abs(out[args.mixed]["energies"][-1] - out["fp64"]["energies"][-1])
     if args.delta == 1.0:
-        exact = -(n - 1) / 4.0
-        out["exact"] = exact
-        out["fp64_abs_err"] = abs(e64 - exact)
-        out["mixed_abs_err"] = abs(emx - exact)
-    log(f"parity: |dE| mixed vs fp64 after {args.parity_sweeps} sweeps {out['abs_dE']:.3e}")
+        out["exact"] = -(n - 1) / 4.0
     return out
-
-
-def sweep_timed(d, direction, label, step):
-    import torch
-
-    t0 = time.perf_counter()
-    n = d.n_sites
-
-    def progress(site, energy):
-        if d._first_site_energy is None:
-            d._first_site_energy = energy
-        done = site + 1 if direction.value == 1 else n - site
-        if done % 20 == 0:
-            log(f"{label} sweep {step}: {done}/{n - 1} sites, E={energy:.12f}, {time.perf_counter() - t0:.1f}s")
-
-    d.on_site = progress
-    d._first_site_energy = None
-    mv0 = d.stats["matvecs"]
-    mv640 = d.stats["matvecs_f64"]
-    e = d.sweep(direction, tol=d._bench_tol)
-    torch.cuda.synchronize()
-    d.on_site = None
-    return e, d.stats["matvecs"] - mv0, d.stats["matvecs_f64"] - mv640, d._first_site_energy
-
-
-def event_time(fn, reps):
-    import torch
-
-    fn()
-    torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        fn()
-    e1.record(s)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps  # ms
-
-
-def kernel_rates(chi, D, d, precisions):
-    """Matvec and dominant-GEMM rates at (chi, D, d), HIP events on the
-    current stream (the one every libtnx call is issued on)."""
-    import torch
-
-    from tnpy_amd import _lib
-
-    L = _lib.lib()
-    dev = torch.device("cuda")
-    g = torch.Generator(device=dev)
-    g.manual_seed(0)
-    Lt = torch.randn(chi, D, chi, dtype=torch.float64, device=dev, generator=g)
-    Rt = torch.randn(chi, D, chi, dtype=torch.float64, device=dev, generator=g)
-    W = torch.randn(D, D, d, d, dtype=torch.float64, device=dev, generator=g)
-    x = torch.randn(chi, d, chi, dtype=torch.float64, device=dev, generator=g)
-    y = torch.empty_like(x)
-    ws = torch.empty(L.tnx_heff1_workspace_bytes(chi, D, D, d, chi), dtype=torch.uint8, device=dev)
-    s = torch.cuda.current_stream().cuda_stream
-    F1 = 4 * D * d * chi**3 + 2 * D * D * d * d * chi * chi
-    A = torch.randn(chi * d, chi, dtype=torch.float64, device=dev, generator=g)
-    B = torch.randn(chi, D * chi, dtype=torch.float64, device=dev, generator=g)
-    C = torch.empty(chi * d, D * chi, dtype=torch.float64, device=dev)
-    FG = 2 * (chi * d) * (D * chi) * chi
-    out = {}
-    for p in precisions:
-        pi = _lib.PRECISIONS[p]
-        tm = event_time(
-            lambda: L.tnx_heff1_matvec(Lt.data_ptr(), W.data_ptr(), Rt.data_ptr(), x.data_ptr(), y.data_ptr(),
-                                       chi, D, D, d, chi, pi, ws.data_ptr(), ws.numel(), s), 10)
-        tg = event_time(
-            lambda: L.tnx_gemm_f64(chi * d, D * chi, chi, 1, 1.0, A.data_ptr(), chi, 0, 1, B.data_ptr(), D * chi,
-                                   0, 1, 0.0, C.data_ptr(), D * chi, 0, pi, s), 20)
-        out[p] = {
-            "matvec_ms": tm,
-            "matvec_tflops": F1 / tm / 1e9,
-            "gemm_ms": tg,
-            "gemm_tflops": FG / tg / 1e9,
-            "gemm_flops_per_launch": FG,
-        }
-    return out
-
-
-def measured_traffic(precision, M, N, K):
-    """HBM bytes per launch of the stage-1 GEMM from the committed counter
-    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by a plain
-    copy's reading), or None when no summary matches this shape."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_stage1_chi1024_traffic.json")
-    try:
-        with open(path) as f:
-            data = json.load(f)
-    except OSError:
-        return None
-    for t in data.get("entries", [data]):
-        if (t.get("precision"), t.get("M"), t.get("N"), t.get("K")) == (precision, M, N, K):
-            return t["bytes_per_launch"]
-    return None
-
-
-def cpu_baseline(chi, D, d, n, mv_per_site, bond, reps):
-    """Bounded CPU sample: the oracle's matvec, SVD gauge move and
-    environment update at chi x chi, each timed ``reps`` times on the host
-    cores, scaled to one sweep of the same chain."""
-    from oracle import dmrg_ref as o
-
-    rng = np.random.default_rng(0)
-    Lt = rng.standard_normal((chi, D, chi))
-    Rt = rng.standard_normal((chi, D, chi))
-    W = o.xxz_bulk(1.0)
-    x = rng.standard_normal((chi, d, chi))
-
-    def best(fn):
-        ts = []
-        for _ in range(reps):
-            t = time.perf_counter()
-            fn()
-            ts.append(time.perf_counter() - t)
-        return min(ts)
-
-    t_mv = best(lambda: o.heff_matvec(Lt, W, Rt, x))
-    arr = [x.copy(), rng.standard_normal((chi, d, chi))]
-    t_svd = best(lambda: o.split_tensor_svd([a.copy() for a in arr], 0, o.RIGHTWARD))
-    t_env = best(lambda: o.env_left(Lt, x, W))
-    # scale per site by the bond dimensions of the chain (cost ~ chil*chir*(chil+chir))
-    dims = [1] + bond + [1]
-    ref = 2.0 * chi**3
-    total = 0.0
-    for i in range(n - 1):
-        l, r = dims[i], dims[i + 1]
-        f = l * r * (l + r) / ref
-        total += f * ((mv_per_site + 1.0) * t_mv + t_env) + (l * d * r * r) / (d * chi**3) * t_svd
-    return {
-        "sweep_s": total,
-        "t_matvec_s": t_mv,
-        "t_svd_s": t_svd,
-        "t_env_s": t_env,
-    }
 
 
 def main():
@@ -418,14 +539,14 @@ def main():
     import torch
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
+    from tnpy_amd import Direction, MatrixProductState, _lib
     from tnpy_amd.model import XXZ
 
     logging.getLogger("tnpy").setLevel(logging.WARNING)
     dev = torch.device("cuda", 0)
-    from tnpy_amd import _lib
-
     _lib.load_library()
+    torch.cuda.synchronize()
+    log("library loaded")
     if args.tiles and os.path.exists(args.tiles):
         log(f"GEMM tile table {args.tiles}: {_lib.lib().tnx_tune_load(args.tiles.encode())} entries")
     if args.save_tiles:
@@ -433,6 +554,8 @@ def main():
 
         atexit.register(lambda: log(f"saved {_lib.lib().tnx_tune_save(args.save_tiles.encode())} tile entries "
                                     f"to {args.save_tiles}"))
+    budget = Budget(args.budget)
+    budget.total = args.budget
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(args, dev)}), flush=True)
         return
@@ -443,67 +566,81 @@ def main():
         rates = kernel_rates(args.chi, 5, 2, ["fp64", args.mixed])
         print(json.dumps({"chi": args.chi, "rates": rates, "copy": copy_reference()}), flush=True)
         return
+
     n, chi = args.n, args.chi
+    # 1. kernel rates at chi
+    rates = kernel_rates(chi, 5, 2, ["fp64", args.mixed])
+    for p in ("fp64", args.mixed):
+        r = rates[p]
+        log(f"{p}: matvec {r['matvec_ms']:.3f} ms ({r['matvec_tflops']:.1f} TF/s), "
+            f"stage-1 GEMM {r['gemm_ms']:.3f} ms ({r['gemm_tflops']:.1f} TF/s)")
+    log(f"gauge move (QR {2 * chi}x{chi} + R*next): {rates['gauge']['ms']:.3f} ms")
+
+    # 2. warm state
     mpo = XXZ(n=n, delta=args.delta).mpo
-    log(f"workload XXZ n={n} delta={args.delta} chi={chi}, building seeded random MPS")
-    t = time.perf_counter()
-    mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=args.seed, device=dev, host_rng=False)
-    torch.cuda.synchronize()
-    log(f"MPS ready in {time.perf_counter() - t:.1f}s; bonds max {mps.bond_dim}")
-
-    def make(prec, state):
-        d = FiniteDMRG(mpo, bond_dim=chi, mps=state, precision=prec, max_basis=args.max_basis,
-                       keep=args.keep, compute_variance=False, device=dev)
-        d._bench_tol = args.tol
-        return d
-
+    state, ramp = warm_state(args, mpo, dev, budget)
+    d64 = make_dmrg(args, mpo, state, "fp64", dev, chi)
     directions = [Direction.RIGHTWARD, Direction.LEFTWARD]
-    d64 = make("fp64", mps)
+    warm = []
     for w in range(args.warmup):
-        e, mv, _, _ = sweep_timed(d64, directions[w % 2], "warmup fp64", w + 1)
-        log(f"warmup sweep {w + 1}: E={e:.12f} matvecs={mv}")
+        dt, e, mv, _ = timed_sweep(d64, directions[w % 2], args.tol, f"warmup {w + 1}")
+        warm.append({"seconds": dt, "energy": e, "matvecs": mv})
+        log(f"warmup sweep {w + 1}: {dt:.2f}s E={e:.14f} matvecs={mv}")
+        if budget.left() < 0.55 * budget.total:
+            break
     snap = [t.clone() for t in d64.mps.tensors]
-    start_dir = args.warmup % 2
-    del d64
+    start_dir = len(warm) % 2
+    bond = d64.mps.bond_dims()
+    del d64, state
     torch.cuda.empty_cache()
 
+    # 3. timed sweeps: FP64 within its share of the budget, then the same
+    # number in the mixed precision
     results = {}
+    K = args.steps
     for prec in ("fp64", args.mixed):
-        d = make(prec, MatrixProductState([t.clone() for t in snap], device=dev))
+        d = make_dmrg(args, mpo, MatrixProductState([t.clone() for t in snap], device=dev), prec, dev, chi)
+        if directions[start_dir] == Direction.LEFTWARD:
+            d.environment.ensure_left(n - 1)  # environment set-up stays outside the timed sweep
         torch.cuda.synchronize()
-        energies, mvs, mv64s, first = [], 0, 0, None
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            e, mv, mv64, e1 = sweep_timed(d, directions[(start_dir + k) % 2], prec, k + 1)
-            first = e1 if first is None else first
+        share_end = time.perf_counter() + 0.25 * budget.total
+        per = []
+        energies, mvs, mv64s = [], 0, 0
+        for k in range(K):
+            dt, e, mv, mv64 = timed_sweep(d, directions[(start_dir + k) % 2], args.tol, f"{prec} sweep {k + 1}")
+            per.append(dt)
             energies.append(e)
             mvs += mv
             mv64s += mv64
+            if prec == "fp64" and time.perf_counter() + dt > share_end:
+                K = k + 1
+                break
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
+        total = float(sum(per))
         results[prec] = {
-            "seconds": dt,
-            "sweeps_per_s": args.steps / dt,
+            "seconds": total,
+            "sweeps": len(per),
+            "sweeps_per_s": len(per) / total,
+            "best3_s": float(np.mean(sorted(per)[:3])),
             "energies": energies,
             "matvecs": mvs,
             "matvecs_f64": mv64s,
             "krylov_sites": d.stats["krylov_sites"],
-            "first_site_energy": first,
         }
-        log(f"{prec}: {args.steps} sweep(s) in {dt:.2f}s, E={energies[-1]:.14f}, matvecs={mvs} (f64 {mv64s})")
-        if prec == "fp64":
-            bond = d.mps.bond_dims()
+        log(f"{prec}: {len(per)} sweep(s) in {total:.2f}s ({len(per) / total:.3f} sweeps/s), "
+            f"E={energies[-1]:.14f}, matvecs={mvs} (f64 {mv64s})")
         del d
         torch.cuda.empty_cache()
-
+    K = results["fp64"]["sweeps"]
     dE = abs(results[args.mixed]["energies"][-1] - results["fp64"]["energies"][-1])
-    dE1 = abs(results[args.mixed]["first_site_energy"] - results["fp64"]["first_site_energy"])
-    log(f"|dE| mixed vs fp64: first site (identical local problem) {dE1:.3e}, end of sweep {dE:.3e}")
-    rates = kernel_rates(chi, 5, 2, ["fp64", args.mixed])
-    for p, r in rates.items():
-        log(f"{p}: matvec {r['matvec_ms']:.3f} ms ({r['matvec_tflops']:.1f} TF/s), "
-            f"GEMM {r['gemm_ms']:.3f} ms ({r['gemm_tflops']:.1f} TF/s)")
+    exact = -(n - 1) / 4.0 if args.delta == 1.0 else None
 
+    # 4. Krylov sample (convergence phase)
+    krylov = None
+    if args.krylov_sites > 0 and budget.left() > 0.2 * budget.total:
+        krylov = krylov_sample(args, mpo, dev, budget)
+
+    # 5. CPU baseline
     cpu = None
     if not args.no_cpu:
         mv_site = results["fp64"]["matvecs"] / max(1, results["fp64"]["krylov_sites"])
@@ -518,59 +655,62 @@ def main():
             "sample": (
                 f"numpy/OpenBLAS FP64 oracle at chi={chi}: best of {args.cpu_reps} of one H_eff matvec "
                 f"({c['t_matvec_s']:.3f}s), one SVD gauge move ({c['t_svd_s']:.3f}s), one environment update "
-                f"({c['t_env_s']:.3f}s); scaled to a sweep with {mv_site:.1f} matvecs/site (the GPU FP64 "
-                f"Lanczos count) and each site's bond dimensions"
+                f"({c['t_env_s']:.3f}s); scaled to a sweep of the timed workload with {mv_site:.2f} matvecs/site "
+                f"(the GPU FP64 count) and each site's bond dimensions"
             ),
         }
         log(f"CPU baseline: {cpu['value']:.3e} sweeps/s ({c['sweep_s']:.1f} s/sweep est.)")
 
+    # 6. many-chain leg
     batched = None
-    if not args.no_batched:
-        batched = batched_leg(args, dev)
-
-    parity = None
-    if not args.no_parity:
-        parity = converged_parity(args, dev)
+    if not args.no_batched and budget.left() > 30.0:
+        batched = batched_leg(args, dev, budget)
+    parity = converged_parity(args, dev) if args.parity else None
 
     mixed = results[args.mixed]
     r = rates[args.mixed]
-    K = args.steps
     line = {
         "metric": METRIC,
         "value": mixed["sweeps_per_s"],
         "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": K,
-        "warmup": args.warmup,
+        "warmup": len(warm),
         "ms_per_step": 1e3 * mixed["seconds"] / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.mixed} matvec + f64 correction",
+        "dtype": f"{args.mixed} Krylov matvecs + f64 residual, f64 env/QR",
         "data": "synthetic (seeded random MPS, XXZ MPO)",
         "config": {
-            "workload": f"XXZ chain n={n} delta={args.delta} chi={chi}, one-site DMRG sweep, tol={args.tol}",
+            "workload": (f"XXZ chain n={n} delta={args.delta} chi={chi}, one-site DMRG sweeps of a converged run "
+                         f"(warm state: DMRG at chi={args.warm_chi}, bonds widened to {chi}, {len(warm)} sweeps "
+                         f"at {chi}), tol={args.tol}"),
             "n": n, "chi": chi, "delta": args.delta, "D": 5, "d": 2, "tol": args.tol, "seed": args.seed,
             "max_basis": args.max_basis, "keep": args.keep,
         },
         "fp64": {
             "sweeps_per_s": results["fp64"]["sweeps_per_s"],
             "ms_per_step": 1e3 * results["fp64"]["seconds"] / K,
+            "best3_ms": 1e3 * results["fp64"]["best3_s"],
             "energy": results["fp64"]["energies"][-1],
             "matvecs": results["fp64"]["matvecs"],
             "matvec_tflops": rates["fp64"]["matvec_tflops"],
         },
         "mixed": {
             "precision": args.mixed,
+            "best3_ms": 1e3 * mixed["best3_s"],
             "energy": mixed["energies"][-1],
-            "abs_dE_first_site": dE1,
-            "abs_dE_end_of_sweep": dE,
+            "abs_dE_vs_fp64": dE,
             "matvecs": mixed["matvecs"],
             "matvecs_f64": mixed["matvecs_f64"],
             "matvec_tflops": r["matvec_tflops"],
             "speedup_vs_fp64": mixed["sweeps_per_s"] / results["fp64"]["sweeps_per_s"],
-            "converged": parity,
         },
+        "exact_energy": exact,
+        "fp64_abs_err_vs_exact": None if exact is None else abs(results["fp64"]["energies"][-1] - exact),
+        "warm": {"ramp": ramp, "sweeps": warm},
+        "krylov_sample": krylov,
         "roofline": {
             "bound": "mfma",
             "kernel": f"gemm_kernel ({args.mixed}) H_eff stage 1, M={2 * chi} N={5 * chi} K={chi}",
@@ -589,8 +729,12 @@ def main():
             "frac": rates["fp64"]["gemm_tflops"] / PEAK_TFLOPS["fp64"],
             "traffic": measured_traffic("fp64", 2 * chi, 5 * chi, chi),
         },
+        "gauge_move": {"ms": rates["gauge"]["ms"], "tflops": rates["gauge"]["tflops"],
+                       "frac_fp64_peak": rates["gauge"]["tflops"] / PEAK_TFLOPS["fp64"]},
         "cpu_baseline": cpu,
         "batched": batched,
+        "parity": parity,
+        "bench_seconds": time.perf_counter() - _T0,
     }
     print(json.dumps(line), flush=True)
 

@@ -61,20 +61,28 @@ int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
                  double beta, double* C, int64_t ldc, int64_t strideC,
                  int precision, tnx_stream_t stream);
 
-/* Runtime tile autotuner of the GEMMs (on by default; TNX_AUTOTUNE=0 in the
- * environment turns it off). The first GEMM of each shape times every tile
- * configuration on a scratch output and caches the fastest for the process.
- * The choice never changes results: only split-K changes the summation
- * order, and it follows a fixed rule on the shape. on=0 also clears the
- * cache. No reference counterpart (quimb leaves this to BLAS). */
+/* GEMM tile choice. Every GEMM takes its tile configuration from the tile
+ * table (shape -> configuration) when the shape is there, else from a fixed
+ * heuristic; neither allocates, synchronises or changes results (only
+ * split-K changes the summation order, and it follows a fixed rule on the
+ * shape). The library's only global state is this table, guarded by a
+ * mutex.
+ * tnx_set_autotune(1) turns on an inline tuning mode for tuning runs (off by
+ * default; TNX_AUTOTUNE=1 in the environment also turns it on): the first
+ * GEMM of a shape missing from the table then times every configuration on a
+ * scratch output it allocates with hipMalloc, waits on the stream, and adds
+ * the fastest to the table. Do not use it under graph capture or where the
+ * caller cannot allow allocation. tnx_tune_clear empties the table.
+ * No reference counterpart (quimb leaves this to BLAS). */
 int tnx_set_autotune(int on);
+int tnx_tune_clear(void);
 
-/* Tile table of the autotuner: tnx_tune_save writes every shape -> tile
- * configuration decided so far in this process to a text file;
+/* Tile table files: tnx_tune_save writes the table to a text file;
  * tnx_tune_load adds a saved table (returns the number of entries, or a
- * negative status), so later processes run the same configurations without
- * timing. TNX_TUNE_CACHE=<file> in the environment loads a table at the
- * first GEMM. No reference counterpart. */
+ * negative status). TNX_TUNE_CACHE=<file> in the environment loads a table
+ * before the first GEMM. tnpy_amd loads its committed MI355X table
+ * (tnpy_amd/gemm_tiles_mi355x.txt) when it loads the library. No reference
+ * counterpart. */
 int tnx_tune_load(const char* path);
 int tnx_tune_save(const char* path);
 

@@ -135,8 +135,11 @@ class TestABI:
         assert "0 2048 5120 1024 1 1 1 0 0 0 1 4" in lines
         assert "1 1024 2048 5120 1 0 1 0 1 0 4 2" in lines
         assert lib.tnx_tune_load(str(tmp_path / "missing.txt").encode()) == -1
-        lib.tnx_set_autotune(0)  # clears the table again
-        lib.tnx_set_autotune(1)
+        lib.tnx_tune_clear()
+        assert lib.tnx_tune_save(str(out).encode()) == 0
+        from tnpy_amd import _lib
+
+        assert lib.tnx_tune_load(str(_lib.TILE_TABLE).encode()) > 0  # restore the committed table
 
     def test_product_raises_without_gpu(self):
         import torch

@@ -16,6 +16,9 @@ from typing import Optional
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libtnx.so"
+# GEMM tile table chosen by the autotuner on MI355X for the shapes of the
+# DMRG path; loaded with the library so no GEMM is timed inline
+TILE_TABLE = Path(__file__).resolve().parent / "gemm_tiles_mi355x.txt"
 
 TNX_F64, TNX_F32, TNX_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": TNX_F64, "fp32": TNX_F32, "bf16x3": TNX_BF16X3}
@@ -57,6 +60,7 @@ SIGNATURES = {
     "tnx_last_error": (C.c_char_p, []),
     "tnx_version": (_i, []),
     "tnx_set_autotune": (_i, [_i]),
+    "tnx_tune_clear": (_i, []),
     "tnx_tune_load": (_i, [C.c_char_p]),
     "tnx_tune_save": (_i, [C.c_char_p]),
     "tnx_gemm_f64": (_i, [_i, _i, _i, _i, _d, _p, _i64, _i64, _i, _p, _i64, _i64, _i, _d, _p, _i64, _i64, _i, _p]),
@@ -128,6 +132,8 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         fn.argtypes = args
     if path is None:
         _LIB = lib
+        if TILE_TABLE.exists():
+            lib.tnx_tune_load(str(TILE_TABLE).encode())
     return lib
 
 

@@ -23,6 +23,7 @@ from __future__ import annotations
 import ctypes as C
 import threading
 import time
+import weakref
 from concurrent.futures import ThreadPoolExecutor
 from itertools import cycle
 from typing import Dict, List, Optional, Sequence
@@ -222,6 +223,8 @@ class _Group:
         return energies
 
     def sweep(self, direction: Direction, tol: float) -> np.ndarray:
+        # worker threads start on the default device; make it this group's
+        torch.cuda.set_device(self.owner.device)
         with torch.cuda.stream(self.stream):
             if direction == Direction.LEFTWARD:
                 self.ensure_left(self.n - 1)
@@ -316,6 +319,23 @@ class BatchedFiniteDMRG:
         torch.cuda.synchronize(self.device)
         self._energies: List[List[float]] = [[] for _ in range(nchain)]
         self.sweep_times: List[float] = []
+        # one worker thread per length group, kept for the life of this
+        # object (created at the first concurrent sweep)
+        self._pool: Optional[ThreadPoolExecutor] = None
+        self._finalizer = None
+
+    def close(self) -> None:
+        """Stop the worker threads of concurrent sweeps."""
+        if self._finalizer is not None:
+            self._finalizer()
+            self._finalizer = None
+        self._pool = None
+
+    def _workers(self) -> ThreadPoolExecutor:
+        if self._pool is None:
+            self._pool = ThreadPoolExecutor(max_workers=len(self._groups), thread_name_prefix="tnx-group")
+            self._finalizer = weakref.finalize(self, self._pool.shutdown, wait=True)
+        return self._pool
 
     def _stream(self) -> int:
         return _lib.stream_ptr(self.device)
@@ -344,10 +364,10 @@ class BatchedFiniteDMRG:
             for g in self._groups:
                 out[g.chains] = g.sweep(direction, tol)
             return out
-        with ThreadPoolExecutor(max_workers=len(self._groups)) as ex:
-            futs = [(g, ex.submit(g.sweep, direction, tol)) for g in self._groups]
-            for g, f in futs:
-                out[g.chains] = f.result()
+        ex = self._workers()
+        futs = [(g, ex.submit(g.sweep, direction, tol)) for g in self._groups]
+        for g, f in futs:
+            out[g.chains] = f.result()
         return out
 
     def run(self, tol: float = 1e-8, max_sweep: int = 100) -> List[List[float]]:

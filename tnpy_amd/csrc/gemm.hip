@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <mutex>
 #include <unordered_map>
 
@@ -138,22 +139,36 @@ struct TuneHash {
 
 static std::mutex g_tune_mu;
 static std::unordered_map<TuneKey, int, TuneHash> g_tune;
-static int g_autotune = -1;  // -1: from TNX_AUTOTUNE (default on)
+// Inline tuning mode: 0 (default) takes the table entry or the heuristic and
+// never allocates or synchronises; 1 times the configurations of a shape the
+// table lacks (allocates a scratch output and waits on the stream).
+static std::atomic<int> g_autotune{0};
+static std::once_flag g_env_once;
 
 static int tune_load_locked(const char* path);
 
-static bool autotune_on() {
-  if (g_autotune < 0) {
+// TNX_AUTOTUNE=1 turns the inline tuning on; TNX_TUNE_CACHE=<file> loads a
+// tile table. Read once, before the first GEMM or tuning call.
+static void init_from_env() {
+  std::call_once(g_env_once, [] {
     const char* e = getenv("TNX_AUTOTUNE");
-    g_autotune = (e && e[0] == '0') ? 0 : 1;
-    // a saved tile table named by TNX_TUNE_CACHE is loaded at first use
+    if (e && e[0] == '1') g_autotune.store(1);
     const char* cache = getenv("TNX_TUNE_CACHE");
     if (cache && cache[0]) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       tune_load_locked(cache);
     }
-  }
-  return g_autotune == 1;
+  });
+}
+
+static thread_local int g_no_tune_depth = 0;
+
+NoTuneScope::NoTuneScope() { ++g_no_tune_depth; }
+NoTuneScope::~NoTuneScope() { --g_no_tune_depth; }
+
+static bool autotune_on() {
+  init_from_env();
+  return g_no_tune_depth == 0 && g_autotune.load() == 1;
 }
 
 // Tile table file: a comment line, then one line per GEMM shape key,
@@ -280,13 +295,18 @@ int gemm(const GemmDesc& d, hipStream_t s) {
 }
 
 void set_autotune(int on) {
+  init_from_env();
+  g_autotune.store(on ? 1 : 0);
+}
+
+void tune_clear() {
+  init_from_env();
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_autotune = on ? 1 : 0;
-  if (!on) g_tune.clear();
+  g_tune.clear();
 }
 
 int tune_load(const char* path) {
-  autotune_on();  // resolve the environment first so it cannot override later
+  init_from_env();  // resolve the environment first so it cannot override later
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const int n = tune_load_locked(path);
   if (n < 0) set_error("tnx_tune_load: cannot read %s", path);
@@ -316,6 +336,11 @@ int tune_save(const char* path) {
 
 extern "C" int tnx_set_autotune(int on) {
   tnx::set_autotune(on);
+  return TNX_OK;
+}
+
+extern "C" int tnx_tune_clear(void) {
+  tnx::tune_clear();
   return TNX_OK;
 }
 

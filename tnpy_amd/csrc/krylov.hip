@@ -23,7 +23,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace tnx {
@@ -1299,43 +1301,60 @@ struct EigCtx {
   double* gsm = nullptr;  // its small arrays (8 x (kMaxBasis + 8))
 };
 
-// Pinned host buffer for the per-step status of the single-problem Lanczos
-// (one per host thread, kept for the life of the process): a copy into it
-// can be waited for with an event while later work stays queued behind it.
-// Null when the allocation fails (the solver then reads its status the
-// plain way).
-static double* pinned_status() {
-  thread_local double* p = nullptr;
-  thread_local bool tried = false;
-  if (!tried) {
-    tried = true;
-    if (hipHostMalloc((void**)&p, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-      p = nullptr;
-      hipGetLastError();
+// Pinned host blocks for the per-step status reads of the Lanczos solvers: a
+// copy into one can be waited for with an event while later work stays
+// queued behind it. A solve leases a block for its duration and returns it;
+// blocks come from one process-wide pool and are never freed (they live
+// until the process ends), so no HIP call runs when a host thread or the
+// process exits, and the pool never holds more blocks than solves ever ran
+// at once. get() is null when the allocation fails (the solver then reads
+// its status the plain way).
+class PinnedLease {
+ public:
+  explicit PinnedLease(size_t n) {
+    {
+      std::lock_guard<std::mutex> lk(mu());
+      auto& fl = free_list();
+      size_t best = fl.size();
+      for (size_t i = 0; i < fl.size(); ++i)
+        if (fl[i].second >= n && (best == fl.size() || fl[i].second < fl[best].second)) best = i;
+      if (best < fl.size()) {
+        p_ = fl[best].first;
+        cap_ = fl[best].second;
+        fl.erase(fl.begin() + (std::ptrdiff_t)best);
+        return;
+      }
     }
+    const size_t want = std::max<size_t>(n, 64);
+    if (hipHostMalloc((void**)&p_, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      p_ = nullptr;
+      hipGetLastError();
+      return;
+    }
+    cap_ = want;
   }
-  return p;
-}
+  ~PinnedLease() {
+    if (!p_) return;
+    std::lock_guard<std::mutex> lk(mu());
+    free_list().emplace_back(p_, cap_);
+  }
+  PinnedLease(const PinnedLease&) = delete;
+  PinnedLease& operator=(const PinnedLease&) = delete;
+  double* get() const { return p_; }
 
-// Pinned host buffer of at least n doubles (one per host thread, grown on
-// demand, kept for the life of the process); null when allocation fails.
-static double* pinned_buffer(size_t n) {
-  thread_local double* p = nullptr;
-  thread_local size_t cap = 0;
-  if (n > cap) {
-    if (p) hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-    const size_t want = std::max<size_t>(n, 1024);
-    if (hipHostMalloc((void**)&p, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-      p = nullptr;
-      hipGetLastError();
-      return nullptr;
-    }
-    cap = want;
+ private:
+  static std::mutex& mu() {
+    static std::mutex m;
+    return m;
   }
-  return p;
-}
+  static std::vector<std::pair<double*, size_t>>& free_list() {
+    // never destroyed: returning a block during process exit stays valid
+    static auto* v = new std::vector<std::pair<double*, size_t>>();
+    return *v;
+  }
+  double* p_ = nullptr;
+  size_t cap_ = 0;
+};
 
 // An event for the duration of one solve.
 struct StepEvent {
@@ -1642,7 +1661,8 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
   // step is discarded (its buffers are rewritten before any later use) and
   // not counted.
   StepEvent sev;
-  double* pin = pinned_status();
+  PinnedLease pin_lease(64);
+  double* pin = pin_lease.get();
   const bool pipelined = sev.e != nullptr && pin != nullptr;
 
   while (true) {
@@ -2021,7 +2041,8 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
   // host reads this step's statuses; problems that finish here drop theirs
   // (not counted), the others take it as their next step.
   StepEvent sev;
-  double* pin = pinned_buffer((size_t)3 * batch);
+  PinnedLease pin_lease((size_t)3 * batch);
+  double* pin = pin_lease.get();
   const bool pipelined = sev.e != nullptr && pin != nullptr;
   auto front = [&](int jj) -> int {
     TNX_TRY(matvec(k.V + (int64_t)jj * N, k.w, cnt));

@@ -89,6 +89,17 @@ struct GemmDesc {
 
 int gemm(const GemmDesc& g, hipStream_t stream);
 void set_autotune(int on);
+void tune_clear();
+
+// While one of these lives on a host thread, the GEMMs that thread issues
+// never enter the inline tuning mode (which allocates and synchronises):
+// they take the tile table entry or the heuristic.
+struct NoTuneScope {
+  NoTuneScope();
+  ~NoTuneScope();
+  NoTuneScope(const NoTuneScope&) = delete;
+  NoTuneScope& operator=(const NoTuneScope&) = delete;
+};
 
 // Elementwise kernels used by several modules.
 int transpose_f64(const double* in, double* out, int rows, int cols,

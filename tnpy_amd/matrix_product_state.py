@@ -201,6 +201,25 @@ class MatrixProductState:
         mps.right_canonicalize()
         return mps
 
+    def grow(self, bond_dim: int) -> "MatrixProductState":
+        """The same state with bonds widened to ``min(d^(i+1), d^(n-i-1),
+        bond_dim)`` (the bond-dimension ramp of a DMRG schedule): each site
+        tensor is zero-padded into the wider shape, then the state is
+        right-canonicalised (Householder LQ completes every padded bond to
+        an orthonormal basis) and normalised. Returns a new MPS."""
+        d = self.phys_dim
+        dims = [1] + _bond_dims(self.n_sites, bond_dim, d) + [1]
+        out = []
+        for i, t in enumerate(self._tensors):
+            l, p, r = t.shape
+            L, R = max(dims[i], l), max(dims[i + 1], r)
+            big = torch.zeros(L, p, R, dtype=torch.float64, device=self.device)
+            big[:l, :, :r] = t
+            out.append(big)
+        mps = MatrixProductState(out, device=self.device)
+        mps.right_canonicalize()
+        return mps
+
     def right_canonicalize(self, normalize: bool = True) -> None:
         """LQ sweep from the right end on the device (tnx_gauge_left)."""
         ws = _lib.Workspace(self.device)
