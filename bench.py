@@ -97,6 +97,8 @@ def parse():
                    help="only time K two-site bond updates at the middle of a random-field Heisenberg "
                         "chain (n=--two-site-n, h=0.5, chi=--chi) in FP64 and in the mixed precision")
     p.add_argument("--two-site-n", type=int, default=128)
+    p.add_argument("--two-site-sample", type=int, default=2, metavar="K",
+                   help="two-site updates per precision in the default run's config-5 sample (0 skips it)")
     p.add_argument("--tiles", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tnpy_amd",
                                                   "gemm_tiles_mi355x.txt"),
                    help="GEMM tile table to load (if it exists) so tile choices repeat across runs")
@@ -405,8 +407,17 @@ def two_site_leg(args, dev):
             f"matvecs {d.stats['matvecs']} (f64 {d.stats['matvecs_f64']}), E={energies[-1]:.12f}")
         del d
         torch.cuda.empty_cache()
+    # the truncation alone: the library SVD of a (chi*d) x (d*chi) block cut to chi
+    theta = torch.randn(chi * 2, 2 * chi, dtype=torch.float64, device=dev)
+    _lib.svd_trunc(theta, chi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, _sv = _lib.svd_trunc(theta, chi)
+    torch.cuda.synchronize()
+    t_svd = time.perf_counter() - t0
     e64, emx = out["fp64"]["energies"], out[args.mixed]["energies"]
     return {
+        "truncation_ms": 1e3 * t_svd,
         "workload": (f"random-field Heisenberg n={n} h=0.5 chi={chi}, {K} two-site updates at the chain middle "
                      f"(sites {mid}..{mid + K - 1}), tol={args.tol}"),
         "fp64": {k: v for k, v in out["fp64"].items() if k != "energies"},
@@ -640,6 +651,12 @@ def main():
     if args.krylov_sites > 0 and budget.left() > 0.2 * budget.total:
         krylov = krylov_sample(args, mpo, dev, budget)
 
+    # 4b. two-site sample (random-field Heisenberg, chi, truncation by the library SVD)
+    two_site = None
+    if args.two_site_sample > 0 and budget.left() > 0.2 * budget.total:
+        args.two_site = args.two_site_sample
+        two_site = two_site_leg(args, dev)
+
     # 5. CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -711,6 +728,7 @@ def main():
         "fp64_abs_err_vs_exact": None if exact is None else abs(results["fp64"]["energies"][-1] - exact),
         "warm": {"ramp": ramp, "sweeps": warm},
         "krylov_sample": krylov,
+        "two_site_sample": two_site,
         "roofline": {
             "bound": "mfma",
             "kernel": f"gemm_kernel ({args.mixed}) H_eff stage 1, M={2 * chi} N={5 * chi} K={chi}",

@@ -201,6 +201,77 @@ int tnx_gauge_left(double* A, const double* prev, double* prev_out,
                    int chil, int d, int chir, int prev_rows, void* ws,
                    size_t ws_bytes, tnx_stream_t stream);
 
+/* ---- truncated SVD ------------------------------------------------------
+ * X (a x b, row-major, device, not modified): the k leading left singular
+ * vectors U (a x k, orthonormal columns, device), Y = U^T X (k x b, device)
+ * and the k leading singular values, descending, to sv_host (host). The
+ * decomposition works on X itself (two QR steps, then one-sided block
+ * Jacobi on the rows of the triangular factor), never on X X^T, so the
+ * singular values are accurate to ~ eps * s_max. sweeps (host, may be NULL)
+ * receives the number of Jacobi sweeps. Synchronises the stream.
+ * Replaces linalg.svd (tnpy/linalg.py:9-23, numpy.linalg.svd cut to
+ * `cutoff` values) applied to the two-site block, the truncation to chi of
+ * two-site DMRG (FiniteDMRG.two_site_solver, a stub in the reference at
+ * tnpy/finite_dmrg.py:113-114), and the Schmidt spectra of
+ * ExactDiagonalization.reduced_density_matrix
+ * (tnpy/exact_diagonalization.py:62-90). */
+size_t tnx_svd_trunc_workspace_bytes(int a, int b, int k);
+int tnx_svd_trunc_f64(const double* X, int a, int b, int k, double* U, double* Y,
+                      double* sv_host, int* sweeps, void* ws, size_t ws_bytes,
+                      tnx_stream_t stream);
+
+/* ---- dense symmetric eigensolver and the dense MPO -----------------------
+ * tnx_syev_f64: every eigenpair of the symmetric n x n matrix A (device, not
+ * modified): eigenvalues ascending to evals_host (host, n), eigenvectors as
+ * the columns of evecs (device, n x n row-major, orthonormal). Computed as
+ * the SVD (tnx_svd_trunc_f64's algorithm) of the positive-definite A + cI,
+ * c = 1 + the largest absolute row sum. Synchronises the stream. Replaces
+ * numpy.linalg.eigh in ExactDiagonalization._eigen_solver
+ * (tnpy/exact_diagonalization.py:58-60).
+ * tnx_mpo_absorb: one MPO site absorbed into the dense operator built so
+ * far, Tout[v][K*d+k][B*d+b] = sum_w T[w][K][B] W[w][v][k][b] (T: Dt x dimK x
+ * dimK, W: Dt x Dr x d x d, Tout: Dr x dimK*d x dimK*d, all device). Site by
+ * site from T = [[[1]]] this is FullHamiltonian.matrix
+ * (tnpy/operators.py:119-154, quimb's MPO contract + fuse). */
+size_t tnx_syev_workspace_bytes(int n);
+int tnx_syev_f64(const double* A, int n, double* evals_host, double* evecs, void* ws,
+                 size_t ws_bytes, tnx_stream_t stream);
+int tnx_mpo_absorb(const double* T, int Dt, int64_t dimK, const double* W, int Dr, int d,
+                   double* Tout, tnx_stream_t stream);
+
+/* ---- shift-invert DMRG: the generalized one-site problem -----------------
+ * Lowest eigenpair of A x = lambda B x with B symmetric positive definite,
+ * x normalised to x^T B x = 1. The reference's ShiftInvertDMRG.one_site_solver
+ * (tnpy/finite_dmrg.py:347-360) passes the one-site operators of the MPO
+ * (H - eps) as A and of its square as B (M=) to linalg.eigh / eigshmv
+ * (tnpy/linalg.py:42-87: scipy.linalg.eigh(a, b) below exact_solver_dim,
+ * primme.eigsh(A, M=B) above).
+ * tnx_sygv_lowest: dense n x n A and B (device): B = W diag(l) W^T
+ * (tnx_syev_f64), S = W diag(l)^-1/2, lowest pair (mu, y) of S^T A S,
+ * x = S y. Synchronises the stream.
+ * tnx_eigsh_gen_heff1: A = H_eff(L, W, R) (bond Dl, Dr), B = H_eff(L2, W2,
+ * R2) (bond D2l, D2r), one-site operators of the same site; generalized
+ * Davidson on a B-orthonormal basis (no preconditioner), stopping when
+ * ||A x - lambda B x|| <= tol (||A x|| + |lambda| ||B x||); thick restart
+ * keeps opts->keep Ritz vectors; FP64 only. hvec (may be NULL) receives A x.
+ * Returns TNX_ERR_NOCONV at the matvec limit. Synchronises the stream. */
+size_t tnx_sygv_lowest_workspace_bytes(int n);
+int tnx_sygv_lowest(const double* A, const double* B, int n, double* x, double* eval_host,
+                    void* ws, size_t ws_bytes, tnx_stream_t stream);
+size_t tnx_eigsh_gen_heff1_workspace_bytes(int chil, int Dl, int Dr, int D2l, int D2r, int d,
+                                           int chir, int max_basis);
+int tnx_eigsh_gen_heff1(const double* L, const double* W, const double* R, const double* L2,
+                        const double* W2, const double* R2, const double* v0, double* vec,
+                        double* hvec, int chil, int Dl, int Dr, int D2l, int D2r, int d,
+                        int chir, const tnx_eigsh_opts_t* opts, tnx_eigsh_info_t* info,
+                        void* ws, size_t ws_bytes, tnx_stream_t stream);
+/* One site of H|phi> as an MPS: out[(l,wl)][k][(r,wr)] =
+ * sum_b W[wl][wr][k][b] A[l][b][r] (device; out is l*Dl x d x r*Dr). The
+ * restored state of ShiftInvertDMRG._restore_mps (tnpy/finite_dmrg.py:316-345,
+ * quimb contract + fuse). */
+int tnx_mpo_apply_site(const double* W, const double* A, int Dl, int Dr, int d, int l, int r,
+                       double* out, tnx_stream_t stream);
+
 /* Householder QR of a row-major m x n matrix, m >= n: A <- Q (m x n,
  * orthonormal columns), R (n x n, upper triangular, row-major).
  * Building block of the gauge moves (numpy.linalg.qr / LAPACK dgeqrf+dorgqr,

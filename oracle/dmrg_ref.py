@@ -422,8 +422,17 @@ def heff_dense(L: np.ndarray, W: np.ndarray, R: np.ndarray) -> np.ndarray:
 def two_site_matvec(L, W1, W2, R, x):
     """Two-site H_eff: y[b,c1,c2,s] = sum L[a,w,b] W1[w,u,k1,c1] W2[u,v,k2,c2]
     R[r,v,s] x[a,k1,k2,r]. No reference function (``finite_dmrg.py:113-114``
-    is a stub); this is the textbook two-site contraction."""
-    return np.einsum("awb,wukc,uvlt,rvs,aklr->bcts", L, W1, W2, R, x, optimize=True).reshape(-1)
+    is a stub); this is the textbook two-site contraction, as four pairwise
+    (BLAS) contractions: R, then W2, then W1, then L."""
+    a, w, b = L.shape
+    r, v, s = R.shape
+    d1, d2 = W1.shape[2], W2.shape[2]
+    X = np.asarray(x).reshape(a, d1, d2, r)
+    T1 = np.tensordot(X, R, axes=([3], [0]))            # a k1 k2 v s
+    T2 = np.tensordot(T1, W2, axes=([2, 3], [2, 1]))    # a k1 s u c2
+    T3 = np.tensordot(T2, W1, axes=([1, 3], [2, 1]))    # a s c2 w c1
+    y = np.tensordot(L, T3, axes=([0, 1], [0, 3]))      # b s c2 c1
+    return y.transpose(0, 3, 2, 1).reshape(-1)
 
 
 def expectation(arrays: List[np.ndarray], mpo: List[np.ndarray]) -> float:

@@ -1,0 +1,296 @@
+"""The benchmark configurations at their real shapes on the GPU, through the
+C-ABI, against the CPU oracle:
+
+* XXZ n=200, delta=1, chi=1024: the one-site H_eff matvec and both
+  environment updates at chi=1024 in FP64 / FP32 / BF16x3, and one Krylov
+  site solve at chi=1024 (FP64 residual recomputed by the oracle; the mixed
+  solver's energy on the identical local problem);
+* random-field Heisenberg n=128, chi=1024, two-site: the two-site matvec at
+  chi=1024;
+* 256 random-coupling chains with lengths from {32, 48, 64, 96}, chi=64: one
+  batched sweep, sampled chains against single-chain FiniteDMRG and the
+  oracle.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import dmrg_ref as o
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+# relative Frobenius error of one chi=1024 contraction in each arithmetic:
+# FP64 to rounding; FP32 and BF16x3 (FP32 accumulation over K = 1024..5120)
+TOL = {"fp64": 1e-13, "fp32": 2e-5, "bf16x3": 2e-4}
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def lib():
+    from tnpy_amd import _lib
+
+    return _lib.lib()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def check(st, what):
+    assert st == 0, f"{what}: status {st}: {lib().tnx_last_error()}"
+
+
+def to_dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def env_operands(chi, D, d, seed):
+    """Environments of a right- and left-canonical random state are
+    symmetric positive in the (ket, bra) pair at w = 0 and D-1; random
+    symmetric blocks stand in for them here."""
+    rng = np.random.default_rng(seed)
+    Lt = rng.standard_normal((chi, D, chi))
+    Rt = rng.standard_normal((chi, D, chi))
+    Lt = 0.5 * (Lt + Lt.transpose(2, 1, 0))
+    Rt = 0.5 * (Rt + Rt.transpose(2, 1, 0))
+    return Lt, Rt
+
+
+@pytest.fixture(autouse=True)
+def quiet_logger():
+    import logging
+
+    lg = logging.getLogger("tnpy")
+    old = lg.level
+    lg.setLevel(logging.WARNING)
+    yield
+    lg.setLevel(old)
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_config3_matvec_chi1024(gpu, prec):
+    chi, D, d = 1024, 5, 2
+    print(f"chi=1024 matvec {prec}", flush=True)
+    Lt, Rt = env_operands(chi, D, d, 3)
+    W = o.xxz_bulk(1.0)
+    x = np.random.default_rng(4).standard_normal((chi, d, chi))
+    y = torch.empty(chi * d * chi, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().tnx_heff1_workspace_bytes(chi, D, D, d, chi), dtype=torch.uint8, device="cuda")
+    Ld, Wd, Rd, xd = to_dev(Lt), to_dev(W), to_dev(Rt), to_dev(x)
+    check(lib().tnx_heff1_matvec(Ld.data_ptr(), Wd.data_ptr(), Rd.data_ptr(), xd.data_ptr(), y.data_ptr(),
+                                 chi, D, D, d, chi, PREC[prec], ws.data_ptr(), ws.numel(), stream()), "matvec")
+    t = time.perf_counter()
+    ref = o.heff_matvec(Lt, W, Rt, x)
+    print(f"oracle matvec {time.perf_counter() - t:.2f}s", flush=True)
+    torch.cuda.synchronize()
+    err = relerr(y.cpu().numpy(), ref)
+    print(f"rel err {err:.2e}", flush=True)
+    assert err < TOL[prec]
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_config3_env_chi1024(gpu, side, prec):
+    chi, D, d = 1024, 5, 2
+    print(f"chi=1024 env {side} {prec}", flush=True)
+    Lt, Rt = env_operands(chi, D, d, 5)
+    W = o.xxz_bulk(1.0)
+    A = np.random.default_rng(6).standard_normal((chi, d, chi)) / np.sqrt(chi)
+    out = torch.empty(chi, D, chi, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().tnx_env_workspace_bytes(chi, D, D, d, chi), dtype=torch.uint8, device="cuda")
+    Ad, Wd = to_dev(A), to_dev(W)
+    if side == "left":
+        Ed = to_dev(Lt)
+        check(lib().tnx_env_left(Ed.data_ptr(), Ad.data_ptr(), Wd.data_ptr(), out.data_ptr(), chi, D, D, d, chi,
+                                 PREC[prec], ws.data_ptr(), ws.numel(), stream()), "env_left")
+        ref = o.env_left(Lt, A, W)
+    else:
+        Ed = to_dev(Rt)
+        check(lib().tnx_env_right(Ed.data_ptr(), Ad.data_ptr(), Wd.data_ptr(), out.data_ptr(), chi, D, D, d, chi,
+                                  PREC[prec], ws.data_ptr(), ws.numel(), stream()), "env_right")
+        ref = o.env_right(Rt, A, W)
+    torch.cuda.synchronize()
+    err = relerr(out.cpu().numpy(), ref)
+    print(f"rel err {err:.2e}", flush=True)
+    assert err < TOL[prec]
+
+
+def _chi1024_site_problem():
+    """A chi=1024 local eigenproblem of the ferromagnetic XXZ chain (delta=1):
+    a 24-site seeded random MPS at chi=1024 (bulk bonds 1024), centre moved
+    to site 11 by QR gauge moves; H_eff of site 11 from the GPU
+    environments."""
+    from tnpy_amd import Direction, Environment, MatrixProductState
+    from tnpy_amd.model import XXZ
+
+    n, chi, site = 24, 1024, 11
+    mps = MatrixProductState.random(n=n, bond_dim=chi, phys_dim=2, seed=12, host_rng=False)
+    for s in range(site):
+        mps.split_tensor(s, Direction.RIGHTWARD)
+    mpo = XXZ(n=n, delta=1.0).mpo
+    env = Environment(mpo, mps)
+    env.ensure_left(site)
+    assert mps.tensors[site].shape == (chi, 2, chi)
+    return env, site
+
+
+def _eigsh(env, site, precision, tol):
+    from tnpy_amd import _lib
+
+    l, Dl, Dr, d, r = env.site_dims(site)
+    N = l * d * r
+    opts = _lib.EigshOpts(tol=tol, max_basis=64, max_matvecs=5000, precision=PREC[precision], keep=20)
+    info = _lib.EigshInfo()
+    vec = torch.empty(N, dtype=torch.float64, device="cuda")
+    hvec = torch.empty(N, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().tnx_eigsh_heff1_workspace_bytes(l, Dl, Dr, d, r, 64), dtype=torch.uint8, device="cuda")
+    v0 = env.mps.tensors[site].reshape(-1).contiguous()
+    st = lib().tnx_eigsh_heff1(env.left[site].data_ptr(), env._W[site].data_ptr(), env.right[site].data_ptr(),
+                               v0.data_ptr(), vec.data_ptr(), hvec.data_ptr(), l, Dl, Dr, d, r, C.byref(opts),
+                               C.byref(info), ws.data_ptr(), ws.numel(), stream())
+    check(st, f"eigsh {precision}")
+    torch.cuda.synchronize()
+    return info, vec, hvec
+
+
+def test_config3_krylov_site_chi1024(gpu):
+    """One chi=1024 site solve (H_eff dimension 2,097,152) in FP64 and in the
+    FP32-mixed solver: the FP64 residual recomputed with one oracle matvec
+    meets tol * |H|_est, H y comes back right, and the two arithmetics agree
+    on the energy of the identical local problem to 1e-10."""
+    tol = 1e-9
+    env, site = _chi1024_site_problem()
+    t = time.perf_counter()
+    i64, v64, h64 = _eigsh(env, site, "fp64", tol)
+    print(f"fp64: E={i64.eval:.14f} matvecs={i64.matvecs} |r|={i64.resnorm:.2e} "
+          f"({time.perf_counter() - t:.2f}s)", flush=True)
+    t = time.perf_counter()
+    imx, vmx, hmx = _eigsh(env, site, "fp32", tol)
+    print(f"fp32-mixed: E={imx.eval:.14f} matvecs={imx.matvecs} (f64 {imx.matvecs_f64}) "
+          f"({time.perf_counter() - t:.2f}s)", flush=True)
+    assert i64.converged == 1 and imx.converged == 1
+    Lh = env.left[site].cpu().numpy()
+    Rh = env.right[site].cpu().numpy()
+    Wh = env._W[site].cpu().numpy()
+    for info, vec, hvec in ((i64, v64, h64), (imx, vmx, hmx)):
+        v = vec.cpu().numpy()
+        hv = o.heff_matvec(Lh, Wh, Rh, v)
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+        res = np.linalg.norm(hv - info.eval * v)
+        # the solver stops on its residual estimate; the true residual of the
+        # returned vector stays within a small factor of it
+        assert res <= 2.0 * tol * info.anorm, (res, tol * info.anorm)
+        assert abs(v @ hv - info.eval) <= 1e-12 * info.anorm
+        np.testing.assert_allclose(hvec.cpu().numpy(), hv, atol=1e-11 * np.abs(hv).max())
+    assert abs(imx.eval - i64.eval) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_config5_two_site_matvec_chi1024(gpu, prec):
+    """The two-site H_eff matvec of the random-field Heisenberg chain at
+    chi=1024 ((1024,2,2,1024) block, F2 = 86.7 GFLOP) against the oracle's
+    two-site contraction."""
+    from tnpy_amd.model import RandomHeisenberg
+
+    chi, d = 1024, 2
+    print(f"chi=1024 two-site matvec {prec}", flush=True)
+    mpo = RandomHeisenberg(n=128, h=0.5, seed=2024).mpo.padded
+    W1, W2 = mpo[63], mpo[64]
+    Dl, Dm, Dr = W1.shape[0], W1.shape[1], W2.shape[1]
+    rng = np.random.default_rng(9)
+    Lt = rng.standard_normal((chi, Dl, chi))
+    Rt = rng.standard_normal((chi, Dr, chi))
+    x = rng.standard_normal((chi, d, d, chi))
+    y = torch.empty(x.size, dtype=torch.float64, device="cuda")
+    ws = torch.empty(lib().tnx_heff2_workspace_bytes(chi, Dl, Dm, Dr, d, chi), dtype=torch.uint8, device="cuda")
+    Ld, W1d, W2d, Rd, xd = (to_dev(a) for a in (Lt, W1, W2, Rt, x))
+    check(lib().tnx_heff2_matvec(Ld.data_ptr(), W1d.data_ptr(), W2d.data_ptr(), Rd.data_ptr(), xd.data_ptr(),
+                                 y.data_ptr(), chi, Dl, Dm, Dr, d, chi, PREC[prec], ws.data_ptr(), ws.numel(),
+                                 stream()), "heff2")
+    t = time.perf_counter()
+    ref = o.two_site_matvec(Lt, W1, W2, Rt, x)
+    print(f"oracle two-site matvec {time.perf_counter() - t:.2f}s", flush=True)
+    torch.cuda.synchronize()
+    err = relerr(y.cpu().numpy(), ref)
+    print(f"rel err {err:.2e}", flush=True)
+    assert err < TOL[prec]
+
+
+def test_config4_256_chains_one_sweep(gpu):
+    """256 random-coupling chains, lengths drawn from {32, 48, 64, 96},
+    chi=64, one rightward sweep in one BatchedFiniteDMRG: one sampled chain
+    per length swept alone by FiniteDMRG from the same state gives the same
+    energies (2e-9 relative: the lockstep solver takes the single solver's
+    iterates, and rounding can move a stopping decision at tol 1e-8), and
+    the shortest sampled chain swept by the oracle (ARPACK in place of the
+    Lanczos solver, same tolerance) ends within 1e-6 of it (the two
+    trajectories from a random state differ by the eigen tolerance at every
+    site)."""
+    from tnpy_amd import Direction, FiniteDMRG
+    from tnpy_amd.batched_dmrg import BatchedFiniteDMRG, random_coupling_chains
+    from tnpy_amd.model import RandomCouplingHeisenberg
+
+    ns, cseeds = random_coupling_chains(256, [32, 48, 64, 96], 7)
+    models = [RandomCouplingHeisenberg(n=n, seed=sd) for n, sd in zip(ns, cseeds)]
+    chi, tol = 64, 1e-8
+    t = time.perf_counter()
+    bd = BatchedFiniteDMRG([m.mpo for m in models], chi=chi, seeds=[700021 + i for i in range(len(ns))],
+                           concurrent=False)
+    assert sorted(bd.groups) == [32, 48, 64, 96] and sum(len(v) for v in bd.groups.values()) == 256
+    sample = [bd.groups[n][0] for n in sorted(bd.groups)]
+    starts = {i: bd.mps(i) for i in sample}
+    eb = bd.sweep(Direction.RIGHTWARD, tol=tol)
+    torch.cuda.synchronize()
+    print(f"batched sweep of 256 chains: {time.perf_counter() - t:.1f}s", flush=True)
+    assert np.all(np.isfinite(eb))
+    for i in sample:
+        f = FiniteDMRG(models[i].mpo, bond_dim=chi, mps=starts[i], compute_variance=False)
+        es = f.sweep(Direction.RIGHTWARD, tol=tol)
+        print(f"chain {i} (n={ns[i]}): batched {eb[i]:.14f} single {es:.14f}", flush=True)
+        assert abs(es - eb[i]) <= 2e-9 * abs(es)
+    i = sample[0]
+    ref = o.OracleDMRG(models[i].mpo.padded, [a.cpu().numpy() for a in starts[i].tensors])
+    t = time.perf_counter()
+    er = ref.sweep(o.RIGHTWARD, tol)
+    print(f"oracle chain {i} (n={ns[i]}): {er:.14f} ({time.perf_counter() - t:.1f}s)", flush=True)
+    assert abs(er - eb[i]) <= 1e-6 * abs(er)
+
+
+def test_config5_truncation_2048(gpu):
+    """The two-site truncation at chi=1024: a 2048 x 2048 block with a
+    decaying spectrum (kept values down to ~1e-10 s_max) cut to 1024 by the
+    library SVD, against numpy.linalg.svd: kept singular values within
+    1e-10 * s_max (measured ~1e-13), orthonormal U, U^T theta exact, and the
+    reconstruction error equal to the discarded weight."""
+    from tnpy_amd import _lib
+
+    n, k = 2048, 1024
+    rng = np.random.default_rng(55)
+    U0, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s0 = np.logspace(0, -14, n)
+    X = (U0 * s0) @ V0.T
+    Xd = to_dev(X)
+    _lib.svd_trunc(Xd, k)  # warm (first-use setup outside the timing)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    U, Y, sv = _lib.svd_trunc(Xd, k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    s = np.linalg.svd(X, compute_uv=False)
+    Uh, Yh = U.cpu().numpy(), Y.cpu().numpy()
+    err = np.abs(sv - s[:k]).max() / s[0]
+    print(f"2048^2 truncation to 1024: {1e3 * dt:.1f} ms, max |ds|/s_max {err:.2e}", flush=True)
+    assert err <= 1e-10
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+    assert np.abs(Yh - Uh.T @ X).max() <= 1e-12 * s[0]
+    resid = np.linalg.norm(X - Uh @ Yh)
+    assert abs(resid - np.sqrt(np.sum(s[k:] ** 2))) <= 1e-10 * s[0]

@@ -301,18 +301,25 @@ def test_qr(gpu, m, n):
     assert relerr(Q @ Rh, A) < 1e-13 * 10
 
 
-def test_qr_rank_deficient(gpu):
-    # rank-3 2000x600 matrix plus exact zero columns: Q must stay orthonormal
+@pytest.mark.parametrize("m, n, rank, zero_cols", [(2000, 600, 3, True), (2048, 1024, 202, False),
+                                                   (2048, 1024, 0, False), (1024, 1024, 40, True)])
+def test_qr_rank_deficient(gpu, m, n, rank, zero_cols):
+    # rank-deficient inputs (the converged ferromagnetic chain's site tensors
+    # have rank ~ n/2 + 1 at chi=1024): Q must stay orthonormal
+    print(f"qr rank-deficient {m}x{n} rank {rank}", flush=True)
     rng = np.random.default_rng(1)
-    A = rng.standard_normal((2000, 3)) @ rng.standard_normal((3, 600))
-    A[:, 100:110] = 0.0
+    A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n)) if rank else np.zeros((m, n))
+    if zero_cols:
+        A[:, 100:110] = 0.0
     Ad = dev(A)
-    R = torch.empty(600, 600, dtype=torch.float64, device="cuda")
-    _, wp, wb = ws(L().tnx_qr_workspace_bytes(2000, 600))
-    check(L().tnx_qr(Ad.data_ptr(), 2000, 600, R.data_ptr(), wp, wb, stream()), "qr")
+    R = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_qr_workspace_bytes(m, n))
+    check(L().tnx_qr(Ad.data_ptr(), m, n, R.data_ptr(), wp, wb, stream()), "qr")
     Q = host(Ad)
-    assert np.linalg.norm(Q.T @ Q - np.eye(600)) < 1e-11
-    assert relerr(Q @ host(R), A) < 1e-12
+    Rh = host(R)
+    assert np.abs(np.tril(Rh, -1)).max() == 0.0
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-11
+    assert np.abs(Q @ Rh - A).max() <= 1e-12 * max(1.0, np.abs(A).max())
 
 
 @pytest.mark.parametrize("chil, d, chir", [(1, 2, 2), (4, 2, 8), (32, 2, 32), (16, 2, 32), (60, 2, 60), (100, 2, 50)])
@@ -424,3 +431,45 @@ def test_heff2_matvec(gpu, prec, shape):
                                dev(x).data_ptr(), y.data_ptr(), *shape, PREC[prec], wp, wb, stream()), "heff2")
     ref = o.two_site_matvec(Lt, W1, W2, Rt, x)
     assert relerr(host(y), ref) < 10 * TOL[prec]
+
+
+# ---------------------------------------------------------------------------
+def _spectrum_matrix(rng, a, b, kind):
+    n = min(a, b)
+    if kind == "random":
+        return rng.standard_normal((a, b))
+    U, _ = np.linalg.qr(rng.standard_normal((a, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((b, n)))
+    if kind == "graded":
+        s = np.logspace(0, -12, n)
+    elif kind == "rank":
+        s = np.concatenate([np.linspace(2.0, 0.5, n // 5), np.zeros(n - n // 5)])
+    else:  # degenerate: Schmidt-like multiplets
+        s = np.repeat(np.logspace(0, -6, max(1, n // 4)), 4)[:n]
+        s = np.concatenate([s, np.zeros(n - len(s))])
+    return (U * s) @ V.T
+
+
+@pytest.mark.parametrize("a, b, k, kind", [(1, 1, 1, "random"), (7, 5, 3, "random"), (37, 23, 10, "random"),
+                                           (23, 37, 10, "random"), (130, 130, 64, "graded"),
+                                           (256, 200, 200, "rank"), (300, 512, 150, "degenerate"),
+                                           (512, 512, 256, "graded")])
+def test_svd_trunc(gpu, a, b, k, kind):
+    """Top-k singular triplets against numpy.linalg.svd: singular values to
+    1e-12 * s_max, U orthonormal, Y = U^T X, and the truncation error equal
+    to the discarded weight (Eckart-Young)."""
+    from tnpy_amd import _lib
+
+    print(f"svd_trunc {a}x{b} k={k} {kind}", flush=True)
+    rng = np.random.default_rng(a * 7 + b)
+    X = _spectrum_matrix(rng, a, b, kind)
+    U, Y, sv = _lib.svd_trunc(dev(X), k)
+    s = np.linalg.svd(X, compute_uv=False)
+    smax = max(s[0], 1e-300)
+    Uh, Yh = host(U), host(Y)
+    assert np.abs(sv - s[:k]).max() <= 1e-12 * smax, np.abs(sv - s[:k]).max() / smax
+    assert np.abs(Uh.T @ Uh - np.eye(k)).max() < 1e-12
+    assert np.abs(Yh - Uh.T @ X).max() <= 1e-12 * smax
+    resid = np.linalg.norm(X - Uh @ Yh)
+    best = np.sqrt(np.sum(s[k:] ** 2))
+    assert abs(resid - best) <= 1e-10 * smax + 1e-8 * best, (resid, best)

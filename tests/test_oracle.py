@@ -319,3 +319,14 @@ class TestSchmidtOracle:
             s2 = ed.reduced_density_matrix(i)
             k = min(len(s2), len(sv[i]))
             np.testing.assert_allclose(sv[i][:k] ** 2, s2[:k], atol=1e-9)
+
+
+def test_two_site_matvec_equals_einsum_definition():
+    """The pairwise-contraction form equals the one-expression definition
+    y = sum L W1 W2 R x (small sizes, where einsum is quick)."""
+    rng = np.random.default_rng(8)
+    L, R = rng.standard_normal((3, 5, 3)), rng.standard_normal((4, 6, 4))
+    W1, W2 = rng.standard_normal((5, 7, 2, 2)), rng.standard_normal((7, 6, 2, 2))
+    x = rng.standard_normal((3, 2, 2, 4))
+    ref = np.einsum("awb,wukc,uvlt,rvs,aklr->bcts", L, W1, W2, R, x).reshape(-1)
+    np.testing.assert_allclose(o.two_site_matvec(L, W1, W2, R, x), ref, atol=1e-12)

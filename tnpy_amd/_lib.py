@@ -13,6 +13,7 @@ import os
 from pathlib import Path
 from typing import Optional
 
+import numpy as np
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libtnx.so"
@@ -103,6 +104,20 @@ SIGNATURES = {
     "tnx_gauge_workspace_bytes": (_sz, [_i, _i, _i]),
     "tnx_gauge_right": (_i, [_p, _p, _p, _i, _i, _i, _i, _p, _sz, _p]),
     "tnx_gauge_left": (_i, [_p, _p, _p, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_svd_trunc_workspace_bytes": (_sz, [_i, _i, _i]),
+    "tnx_svd_trunc_f64": (_i, [_p, _i, _i, _i, _p, _p, C.POINTER(_d), C.POINTER(_i), _p, _sz, _p]),
+    "tnx_syev_workspace_bytes": (_sz, [_i]),
+    "tnx_syev_f64": (_i, [_p, _i, C.POINTER(_d), _p, _p, _sz, _p]),
+    "tnx_mpo_absorb": (_i, [_p, _i, _i64, _p, _i, _i, _p, _p]),
+    "tnx_sygv_lowest_workspace_bytes": (_sz, [_i]),
+    "tnx_sygv_lowest": (_i, [_p, _p, _i, _p, C.POINTER(_d), _p, _sz, _p]),
+    "tnx_eigsh_gen_heff1_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i, _i, _i]),
+    "tnx_eigsh_gen_heff1": (
+        _i,
+        [_p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, C.POINTER(EigshOpts), C.POINTER(EigshInfo),
+         _p, _sz, _p],
+    ),
+    "tnx_mpo_apply_site": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "tnx_qr_workspace_bytes": (_sz, [_i, _i]),
     "tnx_qr": (_i, [_p, _i, _i, _p, _p, _sz, _p]),
     "tnx_axpy": (_i, [_i64, _d, _p, _p, _p]),
@@ -166,6 +181,26 @@ def ptr(t: torch.Tensor, name: str = "tensor") -> int:
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
     return t.data_ptr()
+
+
+def svd_trunc(X: torch.Tensor, k: int, workspace: Optional["Workspace"] = None):
+    """Top-k left singular vectors U (a x k), U^T X (k x b) and the k leading
+    singular values (numpy, descending) of the device matrix X (a x b), by
+    tnx_svd_trunc_f64."""
+    if X.dim() != 2:
+        raise ValueError("svd_trunc needs a matrix")
+    a, b = int(X.shape[0]), int(X.shape[1])
+    k = int(k)
+    X = X.contiguous()
+    ws = workspace if workspace is not None else Workspace(X.device)
+    U = torch.empty(a, k, dtype=torch.float64, device=X.device)
+    Y = torch.empty(k, b, dtype=torch.float64, device=X.device)
+    sv = (C.c_double * k)()
+    sweeps = C.c_int(0)
+    wp, wb = ws.get(lib().tnx_svd_trunc_workspace_bytes(a, b, k))
+    check(lib().tnx_svd_trunc_f64(ptr(X, "X"), a, b, k, ptr(U, "U"), ptr(Y, "Y"), sv, C.byref(sweeps), wp, wb,
+                                  stream_ptr(X.device)), "tnx_svd_trunc_f64")
+    return U, Y, np.frombuffer(sv, dtype=np.float64).copy()
 
 
 class Workspace:

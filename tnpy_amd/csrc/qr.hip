@@ -199,6 +199,39 @@ __global__ void eye_kernel(double* A, int m, int n) {
   }
 }
 
+// Sign convention of every QR here: R with a non-negative diagonal (a
+// column j with R_jj < 0 flips column j of Q and row j of R), so the TSQR
+// panels, the one-workgroup panels and the LDS-resident batched QR return
+// the same factorisation of a full-rank matrix.
+__global__ void qr_signs_kernel(const double* __restrict__ R, int n, double* __restrict__ sg) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    sg[j] = R[(int64_t)j * n + j] < 0.0 ? -1.0 : 1.0;
+}
+
+__global__ void qr_apply_signs_kernel(double* __restrict__ Q, int m, double* __restrict__ R, int n,
+                                      const double* __restrict__ sg) {
+  const int64_t mn = (int64_t)m * n, nn = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn + nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < mn) {
+      Q[e] *= sg[e % n];
+    } else {
+      const int64_t f = e - mn;
+      R[f] *= sg[f / n];
+    }
+  }
+}
+
+static int qr_fix_signs(double* Q, int m, double* R, int n, double* sg, hipStream_t s) {
+  hipLaunchKernelGGL(qr_signs_kernel, dim3((n + 255) / 256), dim3(256), 0, s, R, n, sg);
+  TNX_LAUNCH_CHECK();
+  const int64_t tot = (int64_t)m * n + (int64_t)n * n;
+  const int nb = (int)std::min<int64_t>((tot + 255) / 256, 8192);
+  hipLaunchKernelGGL(qr_apply_signs_kernel, dim3(nb), dim3(256), 0, s, Q, m, R, n, sg);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
 struct QrCfg { int nt, rpt, nb; };
 
 static bool qr_config(int m, QrCfg* c) {
@@ -212,7 +245,330 @@ static bool qr_config(int m, QrCfg* c) {
 
 static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// ---- TSQR panels ---------------------------------------------------------------
+// A panel of b <= kTsNB columns and mp rows is factored without a long
+// serial chain on one workgroup:
+//   leaf  (one workgroup per block of h..2h-1 rows): Householder QR of the
+//         block in LDS, its R_i and explicit thin Q_i;
+//   root  (one workgroup): Householder QR of the stacked R_i, giving the
+//         panel's R and the explicit Q_top, then the Householder vectors of
+//         the whole panel are rebuilt from its explicit Q (Ballard, Demmel,
+//         Grigori, Jacquelin, Nguyen, Solomonik, "Reconstructing Householder
+//         vectors from tall-skinny QR", 2014): with Q1 the top b x b block of
+//         Q, the LU factorisation without pivoting Q1 - S = L1 U, where the
+//         signs S_ii = -sign of the current pivot make |U_ii| >= 1, gives
+//         V = [L1; Q2 U^-1], T = -U S L1^-T and R = S R_tsqr, so that the
+//         panel equals (I - V T V^T)[R; 0] in the compact-WY form the
+//         trailing updates use;
+//   apply (one workgroup per leaf): the explicit Q rows Q_i Q_top,i, then
+//         V rows (Q2 U^-1), V T^T and V T for the two-GEMM trailing update
+//         A_trail -= (V T^T)(V^T A_trail) and the backward formation of Q,
+//         Q_sub -= (V T)(V^T Q_sub).
+constexpr int kTsNB = 32;         // panel width
+constexpr int kTsLd = kTsNB + 1;  // LDS row stride
+constexpr int kTsMaxRows = 511;   // rows of one leaf (h <= 256, last leaf < 2h)
+constexpr int kTsMaxLeaves = 16;  // stacked R_i in the root: 16 * 32 rows
+
+struct TsPlan { int h, nleaf; };
+
+static bool ts_plan(int mp, TsPlan* p) {
+  const int h = mp > kTsMaxLeaves * 128 + 127 ? 256 : 128;
+  const int nleaf = std::max(1, mp / h);
+  if (nleaf > kTsMaxLeaves || mp - (nleaf - 1) * h > kTsMaxRows) return false;
+  *p = {h, nleaf};
+  return true;
+}
+
+template <int NT>
+__device__ __forceinline__ double ts_block_sum(double v, double* red) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wsum(v);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) t += red[w];
+  return t;
+}
+
+// Householder QR of the m x n (m >= n) matrix held in LDS (row stride ld):
+// R on and above the diagonal, the reflector tails below it, tau_s[j].
+template <int NT>
+__device__ void lds_householder(double* S, int ld, int m, int n, double* tau_s, double* red) {
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = 0; j < n; ++j) {
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += NT) part += S[i * ld + j] * S[i * ld + j];
+    const double sigma = ts_block_sum<NT>(part, red);
+    const double alpha = S[j * ld + j];
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (sigma != 0.0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < m; i += NT) S[i * ld + j] *= scale;
+    if (tid == 0) { S[j * ld + j] = beta; tau_s[j] = tj; }
+    __syncthreads();
+    if (tj != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += NW) {
+        double acc = 0.0;
+        for (int i = j + lane; i < m; i += 64) acc += (i == j ? 1.0 : S[i * ld + j]) * S[i * ld + c];
+        const double w = tj * wsum(acc);
+        for (int i = j + lane; i < m; i += 64) S[i * ld + c] -= (i == j ? 1.0 : S[i * ld + j]) * w;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Explicit thin Q = H_0 ... H_{n-1} [I; 0], formed in place over the output
+// of lds_householder (whose R is overwritten).
+template <int NT>
+__device__ void lds_form_q(double* S, int ld, int m, int n, const double* tau_s) {
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = n - 1; j >= 0; --j) {
+    const double tj = tau_s[j];
+    if (tj != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += NW) {
+        double acc = 0.0;
+        for (int i = j + lane; i < m; i += 64) acc += (i == j ? 1.0 : S[i * ld + j]) * S[i * ld + c];
+        const double w = tj * wsum(acc);
+        for (int i = j + lane; i < m; i += 64) S[i * ld + c] -= (i == j ? 1.0 : S[i * ld + j]) * w;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += NT)
+      S[i * ld + j] = (i < j) ? 0.0 : (i == j ? 1.0 - tj : -tj * S[i * ld + j]);
+    __syncthreads();
+  }
+}
+
+// Leaf: rows [leaf*h, leaf*h + rows) of the panel A[jb:, jb:jb+b].
+// Writes R_i (kTsNB x kTsNB, zero padded) to Rst and the explicit Q_i rows to
+// Qst (mp x kTsNB, panel-relative rows).
+__global__ __launch_bounds__(256) void tsqr_leaf_kernel(const double* __restrict__ A, int n, int jb,
+                                                       int mp, int b, int h, int nleaf,
+                                                       double* __restrict__ Rst,
+                                                       double* __restrict__ Qst) {
+  __shared__ double S[kTsMaxRows * kTsLd];
+  __shared__ double tau_s[kTsNB];
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const int leaf = blockIdx.x;
+  const int r0 = leaf * h;
+  const int rows = leaf == nleaf - 1 ? mp - r0 : h;
+  for (int e = tid; e < rows * b; e += 256) {
+    const int i = e / b, c = e - i * b;
+    S[i * kTsLd + c] = A[(int64_t)(jb + r0 + i) * n + jb + c];
+  }
+  __syncthreads();
+  lds_householder<256>(S, kTsLd, rows, b, tau_s, red);
+  double* Rp = Rst + (size_t)leaf * kTsNB * kTsNB;
+  for (int e = tid; e < kTsNB * kTsNB; e += 256) {
+    const int i = e / kTsNB, c = e - i * kTsNB;
+    Rp[e] = (i < b && c < b && c >= i) ? S[i * kTsLd + c] : 0.0;
+  }
+  __syncthreads();
+  lds_form_q<256>(S, kTsLd, rows, b, tau_s);
+  for (int e = tid; e < rows * kTsNB; e += 256) {
+    const int i = e / kTsNB, c = e - i * kTsNB;
+    Qst[(int64_t)(r0 + i) * kTsNB + c] = c < b ? S[i * kTsLd + c] : 0.0;
+  }
+}
+
+// Root: QR of the stacked R_i, explicit Q_top, then the reconstruction of the
+// panel's Householder form (see above). Writes the panel's R (= S R_tsqr)
+// into A's panel rows on and above the diagonal, L1 into Vall's panel rows
+// 0..b-1, T, U and Q_top for the apply kernel.
+__global__ __launch_bounds__(512) void tsqr_root_kernel(const double* __restrict__ Rst, int nleaf,
+                                                       int b, const double* __restrict__ Qst,
+                                                       double* __restrict__ A, int n, int jb,
+                                                       double* __restrict__ Vall,
+                                                       double* __restrict__ Tst,
+                                                       double* __restrict__ Qtop,
+                                                       double* __restrict__ Ust) {
+  __shared__ double S[kTsMaxLeaves * kTsNB * kTsLd];
+  __shared__ double Rb[kTsNB * kTsLd];
+  __shared__ double B[kTsNB * kTsLd];
+  __shared__ double Li[kTsNB * kTsLd];
+  __shared__ double tau_s[kTsNB];
+  __shared__ double sg[kTsNB];
+  __shared__ double red[8];
+  const int tid = threadIdx.x;
+  const int rows = nleaf * b;
+  for (int e = tid; e < rows * b; e += 512) {
+    const int r = e / b, c = e - r * b;
+    const int k = r / b, i = r - k * b;
+    S[r * kTsLd + c] = Rst[(size_t)k * kTsNB * kTsNB + i * kTsNB + c];
+  }
+  __syncthreads();
+  lds_householder<512>(S, kTsLd, rows, b, tau_s, red);
+  for (int e = tid; e < b * b; e += 512) {
+    const int i = e / b, c = e - i * b;
+    Rb[i * kTsLd + c] = c >= i ? S[i * kTsLd + c] : 0.0;
+  }
+  __syncthreads();
+  lds_form_q<512>(S, kTsLd, rows, b, tau_s);
+  for (int e = tid; e < rows * kTsNB; e += 512) {
+    const int r = e / kTsNB, c = e - r * kTsNB;
+    Qtop[e] = c < b ? S[r * kTsLd + c] : 0.0;
+  }
+  // Q1 = (leaf 0's Q rows 0..b-1) x (Q_top rows 0..b-1)
+  for (int e = tid; e < b * b; e += 512) {
+    const int i = e / b, c = e - i * b;
+    double acc = 0.0;
+    for (int k = 0; k < b; ++k) acc += Qst[i * kTsNB + k] * S[k * kTsLd + c];
+    B[i * kTsLd + c] = acc;
+  }
+  __syncthreads();
+  // LU of Q1 - S without pivoting, S chosen at each pivot: |U_ii| >= 1
+  for (int i = 0; i < b; ++i) {
+    if (tid == 0) {
+      const double a = B[i * kTsLd + i];
+      const double sgn = a >= 0.0 ? -1.0 : 1.0;
+      B[i * kTsLd + i] = a - sgn;
+      sg[i] = sgn;
+    }
+    __syncthreads();
+    const double piv = B[i * kTsLd + i];
+    for (int r = i + 1 + tid; r < b; r += 512) B[r * kTsLd + i] /= piv;
+    __syncthreads();
+    const int w = b - 1 - i;
+    for (int e = tid; e < w * w; e += 512) {
+      const int r = i + 1 + e / w, c = i + 1 + e % w;
+      B[r * kTsLd + c] -= B[r * kTsLd + i] * B[i * kTsLd + c];
+    }
+    __syncthreads();
+  }
+  // Li = L1^{-1} (unit lower), one column per thread
+  if (tid < b) {
+    const int c = tid;
+    for (int r = 0; r < b; ++r) {
+      double x;
+      if (r < c) x = 0.0;
+      else if (r == c) x = 1.0;
+      else {
+        double acc = 0.0;
+        for (int k = c; k < r; ++k) acc += B[r * kTsLd + k] * Li[k * kTsLd + c];
+        x = -acc;
+      }
+      Li[r * kTsLd + c] = x;
+    }
+  }
+  __syncthreads();
+  // T = -U S L1^-T ; U ; L1 -> Vall ; R -> A
+  for (int e = tid; e < kTsNB * kTsNB; e += 512) {
+    const int i = e / kTsNB, j = e - i * kTsNB;
+    double t = 0.0, u = 0.0;
+    if (i < b && j < b && i <= j) {
+      for (int k = i; k <= j; ++k) t += B[i * kTsLd + k] * sg[k] * Li[j * kTsLd + k];
+      u = B[i * kTsLd + j];
+    }
+    Tst[e] = -t;
+    Ust[e] = u;
+  }
+  for (int e = tid; e < b * b; e += 512) {
+    const int i = e / b, c = e - i * b;
+    const int64_t off = (int64_t)(jb + i) * n + jb + c;
+    Vall[off] = c < i ? B[i * kTsLd + c] : (c == i ? 1.0 : 0.0);
+    if (c >= i) A[off] = sg[i] * Rb[i * kTsLd + c];
+  }
+}
+
+// Apply: V rows below the top b rows = Q_i Q_top,i U^-1 = Q_i M with
+// M = Q_top,i U^-1 solved once per workgroup (the top rows are L1, from the
+// root); Y1 = V T^T, Y2 = V T. One panel row per thread.
+__global__ __launch_bounds__(256) void tsqr_apply_kernel(const double* __restrict__ Qst,
+                                                        const double* __restrict__ Qtop,
+                                                        const double* __restrict__ Ust,
+                                                        const double* __restrict__ Tst, int mp,
+                                                        int b, int h, int nleaf, int n, int jb,
+                                                        double* __restrict__ Vall,
+                                                        double* __restrict__ Y1,
+                                                        double* __restrict__ Y2) {
+  __shared__ double M[kTsNB * kTsNB];
+  __shared__ double U[kTsNB * kTsNB];
+  __shared__ double T[kTsNB * kTsNB];
+  const int tid = threadIdx.x;
+  const int leaf = blockIdx.x;
+  const int r0 = leaf * h;
+  const int rows = leaf == nleaf - 1 ? mp - r0 : h;
+  for (int e = tid; e < kTsNB * kTsNB; e += 256) {
+    const int k = e / kTsNB;
+    M[e] = k < b ? Qtop[(size_t)(leaf * b + k) * kTsNB + (e - k * kTsNB)] : 0.0;
+    U[e] = Ust[e];
+    T[e] = Tst[e];
+  }
+  __syncthreads();
+  // row k of M: M[k] U = Q_top,i[k]  (U upper triangular, |U_cc| >= 1)
+  if (tid < b) {
+    const int k = tid;
+    for (int c = 0; c < b; ++c) {
+      double acc = M[k * kTsNB + c];
+      for (int j = 0; j < c; ++j) acc -= M[k * kTsNB + j] * U[j * kTsNB + c];
+      M[k * kTsNB + c] = acc / U[c * kTsNB + c];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < rows; i += 256) {
+    const int pr = r0 + i;
+    const int64_t off = (int64_t)(jb + pr) * n + jb;
+    double v[kTsNB];
+    if (pr < b) {
+#pragma unroll
+      for (int c = 0; c < kTsNB; ++c) v[c] = c < b ? Vall[off + c] : 0.0;
+    } else {
+#pragma unroll
+      for (int c = 0; c < kTsNB; ++c) v[c] = 0.0;
+      const double* qrow = Qst + (int64_t)pr * kTsNB;
+      for (int k = 0; k < b; ++k) {
+        const double a = qrow[k];
+#pragma unroll
+        for (int c = 0; c < kTsNB; ++c) v[c] += a * M[k * kTsNB + c];
+      }
+#pragma unroll
+      for (int c = 0; c < kTsNB; ++c)
+        if (c < b) Vall[off + c] = v[c];
+    }
+    for (int c = 0; c < b; ++c) {
+      double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < kTsNB; ++k) {
+        y1 += v[k] * T[c * kTsNB + k];  // (V T^T)[r][c] = sum_k V[r][k] T[c][k]
+        y2 += v[k] * T[k * kTsNB + c];  // (V T)[r][c]   = sum_k V[r][k] T[k][c]
+      }
+      Y1[off + c] = y1;
+      Y2[off + c] = y2;
+    }
+  }
+}
+
+static bool ts_fits(int m, int n) {
+  TsPlan p;
+  return m >= n && ts_plan(m, &p);  // the first panel has the most rows
+}
+
+static size_t ts_ws_bytes(int m, int n) {
+  size_t b = 3 * al((size_t)m * n * 8);                  // Vall, Y1 = V T^T, Y2 = V T
+  b += al((size_t)m * kTsNB * 8);                        // leaf explicit Q rows
+  b += al((size_t)kTsMaxLeaves * kTsNB * kTsNB * 8);     // leaf R factors
+  b += al((size_t)kTsMaxLeaves * kTsNB * kTsNB * 8);     // Q_top
+  b += 2 * al((size_t)kTsNB * kTsNB * 8);                // T, U
+  b += al((size_t)kTsNB * n * 8);                        // W
+  b += al((size_t)32 * kTsNB * n * 8);                   // split-K partials
+  b += al((size_t)n * 8);                                // R diagonal signs
+  return b + 256;
+}
+
 size_t qr_ws_bytes(int m, int n) {
+  if (ts_fits(m, n)) return ts_ws_bytes(m, n);
   QrCfg c;
   if (!qr_config(m, &c)) c = {1024, 8, 4};
   const int np = (n + c.nb - 1) / c.nb;
@@ -220,6 +576,7 @@ size_t qr_ws_bytes(int m, int n) {
   b += al((size_t)np * c.nb * c.nb * 8);            // T factors
   b += 2 * al((size_t)c.nb * n * 8);                // W, W2
   b += al((size_t)32 * c.nb * n * 8);               // split-K partials
+  b += al((size_t)n * 8);                           // R diagonal signs
   return b + 256;
 }
 
@@ -242,9 +599,109 @@ static int launch_panel(const QrCfg& c, double* A, int m, int n, int jb, int b, 
   return TNX_ERR_UNSUPPORTED;
 }
 
+// Blocked QR with TSQR panels: per panel three small kernels (leaf, root,
+// apply) and two GEMMs for the trailing update; Q formed backwards with two
+// GEMMs per panel.
+static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) {
+  char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  double* Vall = (double*)p; p += al((size_t)m * n * 8);
+  double* Y1 = (double*)p; p += al((size_t)m * n * 8);
+  double* Y2 = (double*)p; p += al((size_t)m * n * 8);
+  double* Qst = (double*)p; p += al((size_t)m * kTsNB * 8);
+  double* Rst = (double*)p; p += al((size_t)kTsMaxLeaves * kTsNB * kTsNB * 8);
+  double* Qtop = (double*)p; p += al((size_t)kTsMaxLeaves * kTsNB * kTsNB * 8);
+  double* Tst = (double*)p; p += al((size_t)kTsNB * kTsNB * 8);
+  double* Ust = (double*)p; p += al((size_t)kTsNB * kTsNB * 8);
+  double* Wb = (double*)p; p += al((size_t)kTsNB * n * 8);
+  double* part = (double*)p; p += al((size_t)32 * kTsNB * n * 8);
+  double* sg = (double*)p;
+  const size_t part_bytes = al((size_t)32 * kTsNB * n * 8);
+  const int np = (n + kTsNB - 1) / kTsNB;
+  for (int pi = 0; pi < np; ++pi) {
+    const int jb = pi * kTsNB;
+    const int b = std::min(kTsNB, n - jb);
+    const int mp = m - jb;
+    TsPlan pl;
+    if (!ts_plan(mp, &pl)) {
+      set_error("qr: panel of %d rows has no TSQR plan", mp);
+      return TNX_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b, pl.h,
+                       pl.nleaf, Rst, Qst);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tsqr_root_kernel, dim3(1), dim3(512), 0, s, Rst, pl.nleaf, b, Qst, A, n, jb,
+                       Vall, Tst, Qtop, Ust);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tsqr_apply_kernel, dim3(pl.nleaf), dim3(256), 0, s, Qst, Qtop, Ust, Tst, mp,
+                       b, pl.h, pl.nleaf, n, jb, Vall, Y1, Y2);
+    TNX_LAUNCH_CHECK();
+    const int nt = n - jb - b;
+    if (nt <= 0) continue;
+    const double* Vp = Vall + (size_t)jb * n + jb;
+    const double* Y1p = Y1 + (size_t)jb * n + jb;
+    double* At = A + (size_t)jb * n + jb + b;
+    // W = V^T A_trail   (b x nt)
+    GemmDesc g{};
+    g.M = b; g.N = nt; g.K = mp; g.batch = 1;
+    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+    g.B = At; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+    g.C = Wb; g.tc = DT::F64; g.ldc = nt; g.sC = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+    TNX_TRY(gemm(g, s));
+    // A_trail -= (V T^T) W
+    GemmDesc g3{};
+    g3.M = mp; g3.N = nt; g3.K = b; g3.batch = 1;
+    g3.A = Y1p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+    g3.B = Wb; g3.tb = DT::F64; g3.ldb = nt; g3.sB = 0; g3.b_ncontig = true;
+    g3.C = At; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+    TNX_TRY(gemm(g3, s));
+  }
+  {
+    const int64_t nn = (int64_t)n * n;
+    const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 4096);
+    hipLaunchKernelGGL(extract_r_kernel, dim3(blocks), dim3(256), 0, s, A, n, R);
+    TNX_LAUNCH_CHECK();
+    const int64_t mn = (int64_t)m * n;
+    const int b2 = (int)std::min<int64_t>((mn + 255) / 256, 4096);
+    hipLaunchKernelGGL(eye_kernel, dim3(b2), dim3(256), 0, s, A, m, n);
+    TNX_LAUNCH_CHECK();
+  }
+  for (int pi = np - 1; pi >= 0; --pi) {
+    const int jb = pi * kTsNB;
+    const int b = std::min(kTsNB, n - jb);
+    const int mp = m - jb;
+    const int nq = n - jb;
+    const double* Vp = Vall + (size_t)jb * n + jb;
+    const double* Y2p = Y2 + (size_t)jb * n + jb;
+    double* Qs = A + (size_t)jb * n + jb;
+    GemmDesc g{};
+    g.M = b; g.N = nq; g.K = mp; g.batch = 1;
+    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+    g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+    g.C = Wb; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+    TNX_TRY(gemm(g, s));
+    GemmDesc g3{};
+    g3.M = mp; g3.N = nq; g3.K = b; g3.batch = 1;
+    g3.A = Y2p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+    g3.B = Wb; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+    g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+    TNX_TRY(gemm(g3, s));
+  }
+  return qr_fix_signs(A, m, R, n, sg, s);
+}
+
 int qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, hipStream_t s) {
   TNX_CHECK_ARG(A && R, "qr: null pointer");
   TNX_CHECK_ARG(m >= 1 && n >= 1 && m >= n, "qr: need m >= n >= 1 (m=%d n=%d)", m, n);
+  if (ts_fits(m, n)) {
+    TNX_CHECK_ARG(ws && ws_bytes >= qr_ws_bytes(m, n), "qr: workspace too small");
+    return qr_tsqr(A, m, n, R, ws, s);
+  }
   QrCfg c;
   if (!qr_config(m, &c)) {
     set_error("qr: m=%d rows exceeds the supported 8192", m);
@@ -258,7 +715,8 @@ int qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, hipStream_
   double* Tall = (double*)p; p += al((size_t)np * NB * NB * 8);
   double* Wb = (double*)p; p += al((size_t)NB * n * 8);
   double* W2 = (double*)p; p += al((size_t)NB * n * 8);
-  double* part = (double*)p;
+  double* part = (double*)p; p += al((size_t)32 * NB * n * 8);
+  double* sg = (double*)p;
   const size_t part_bytes = al((size_t)32 * NB * n * 8);
 
   // ---- factorisation
@@ -340,7 +798,7 @@ int qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, hipStream_
     g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
     TNX_TRY(gemm(g3, s));
   }
-  return TNX_OK;
+  return qr_fix_signs(A, m, R, n, sg, s);
 }
 
 // ---- gauge moves -------------------------------------------------------------
@@ -458,11 +916,14 @@ __global__ __launch_bounds__(256) void qr_small_kernel(double* __restrict__ A, i
     }
     __syncthreads();
   }
-  // ---- R
+  // ---- R, with a non-negative diagonal (same convention as the blocked QR)
+  __shared__ double sgn[256];
+  for (int j = tid; j < n; j += 256) sgn[j] = S[j * ld + j] < 0.0 ? -1.0 : 1.0;
+  __syncthreads();
   double* Rp = R + (int64_t)blockIdx.x * n * n;
   for (int e = tid; e < n * n; e += 256) {
     const int i = e / n, c = e - i * n;
-    Rp[e] = (c >= i) ? S[i * ld + c] : 0.0;
+    Rp[e] = (c >= i) ? sgn[i] * S[i * ld + c] : 0.0;
   }
   __syncthreads();
   // ---- thin Q = H_0 ... H_{n-1} [I; 0], formed in place from the last column
@@ -484,7 +945,7 @@ __global__ __launch_bounds__(256) void qr_small_kernel(double* __restrict__ A, i
   for (int e = tid; e < mn; e += 256) {
     int i, c;
     if (trans) { c = e / m; i = e - c * m; } else { i = e / n; c = e - i * n; }
-    Ap[e] = S[i * ld + c];
+    Ap[e] = sgn[c] * S[i * ld + c];
   }
 }
 

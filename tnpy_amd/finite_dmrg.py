@@ -268,35 +268,27 @@ class FiniteDMRG:
         e, theta = self._solve2(site, tol, **kwargs)
         return e, theta.cpu().numpy()
 
-    def _split2(self, site: int, theta: torch.Tensor, direction: Direction) -> torch.Tensor:
-        """Truncated split of theta into two sites keeping
-        min(chi, l*d, d*r) states: the dominant eigenvectors of the reduced
-        density matrix theta theta^T (rightward: site <- U, next <- U^T theta)
-        or theta^T theta (leftward: next <- V^T, site <- theta V). Returns the
-        kept weights (squared singular values, descending)."""
+    def _split2(self, site: int, theta: torch.Tensor, direction: Direction) -> np.ndarray:
+        """Truncated split of theta into two sites keeping min(chi, l*d, d*r)
+        singular values (the SVD cut of ``tnpy/linalg.py:9-23``), by the
+        library's truncated SVD of theta itself: rightward site <- U_k,
+        next <- U_k^T theta; leftward next <- V_k^T, site <- theta V_k.
+        Returns the kept singular values (descending)."""
         l, d1, d2, r = theta.shape
         m, n = l * d1, d2 * r
         k = min(self.bond_dim, m, n)
         M = theta.reshape(m, n)
+        ws = self._env.workspace
         if direction == Direction.RIGHTWARD:
-            rho = torch.empty(m, m, dtype=torch.float64, device=self.device)
-            self._gemm(m, m, n, M, n, True, M, n, False, rho, m)
-            w, U = torch.linalg.eigh(rho)
-            U = U[:, m - k:].flip(-1).contiguous()
-            Bn = torch.empty(k, n, dtype=torch.float64, device=self.device)
-            self._gemm(k, n, m, U, k, False, M, n, True, Bn, n)
+            U, Y, s = _lib.svd_trunc(M, k, ws)
             self.mps._tensors[site] = U.reshape(l, d1, k)
-            self.mps._tensors[site + 1] = Bn.reshape(k, d2, r)
+            self.mps._tensors[site + 1] = Y.reshape(k, d2, r)
         else:
-            rho = torch.empty(n, n, dtype=torch.float64, device=self.device)
-            self._gemm(n, n, m, M, n, False, M, n, True, rho, n)
-            w, V = torch.linalg.eigh(rho)
-            V = V[:, n - k:].flip(-1).contiguous()
-            An = torch.empty(m, k, dtype=torch.float64, device=self.device)
-            self._gemm(m, k, n, M, n, True, V, k, True, An, k)
-            self.mps._tensors[site] = An.reshape(l, d1, k)
+            V, Yt, s = _lib.svd_trunc(M.T.contiguous(), k, ws)
+            self.mps._tensors[site] = Yt.T.contiguous().reshape(l, d1, k)
             self.mps._tensors[site + 1] = V.T.contiguous().reshape(k, d2, r)
-        return w.flip(-1)[:k]
+        self.last_singular_values = s
+        return s
 
     def _two_site_step(self, site: int, direction: Direction, tol: float, **kwargs) -> float:
         energy, theta = self._solve2(site, tol, **kwargs)
@@ -426,3 +418,168 @@ class FiniteDMRG:
         if len(self._energies) == 1:
             raise RuntimeError("FiniteDMRG is probably not executed yet.")
         return MatrixProductStateMeasurements(self.mps)
+
+
+class ShiftInvertDMRG(FiniteDMRG):
+    """DMRG on the shift-invert spectrum (``tnpy/finite_dmrg.py:266-407``):
+
+        (H - eps) phi_n = 1 / (E_n - eps) (H - eps)^2 phi_n,
+
+    optimised for the lowest 1 / (E - eps), i.e. the eigenstate just below
+    the offset eps; the eigenstate itself is restored as psi = (H - eps) phi.
+
+    Args:
+        mpo: the MPO of H - eps (e.g. ``RandomHeisenberg(..., offset=eps)``).
+        bond_dim: bond dimension (``chi`` is an alias).
+        offset: eps.
+        block_size, mps, exact_solver_dim: as for ``FiniteDMRG`` (one-site
+            updates only).
+
+    Per site the generalized eigenproblem is solved on the GPU: dense
+    (``tnx_sygv_lowest``) below ``exact_solver_dim``, generalized Davidson
+    (``tnx_eigsh_gen_heff1``) above, with a second environment of the
+    squared MPO (D^2 bond) kept next to the first.
+    """
+
+    def __init__(
+        self,
+        mpo,
+        bond_dim: Optional[int] = None,
+        offset: float = 0,
+        block_size: int = 1,
+        mps=None,
+        exact_solver_dim: int = 200,
+        *,
+        chi: Optional[int] = None,
+        seed: Optional[int] = None,
+        max_basis: int = 64,
+        keep: int = 8,
+        compute_variance: bool = True,
+        device: Optional[torch.device] = None,
+    ):
+        if block_size != 1:
+            raise ValueError("ShiftInvertDMRG runs one-site updates (block_size=1)")
+        super().__init__(
+            mpo, bond_dim=bond_dim, block_size=1, mps=mps, exact_solver_dim=exact_solver_dim, chi=chi,
+            precision="fp64", seed=seed, max_basis=max_basis, keep=keep, compute_variance=compute_variance,
+            device=device,
+        )
+        self._env2 = Environment(mpo=self._env.mpo.square(), mps=self.mps)
+        self._offset = offset
+        self._restored_mps: Optional[MatrixProductState] = None
+
+    @property
+    def offset(self) -> float:
+        return self._offset
+
+    @property
+    def restored_mps(self) -> Optional[MatrixProductState]:
+        return self._restored_mps
+
+    def _solve(self, site: int, tol: float, **kwargs) -> Tuple[float, torch.Tensor, torch.Tensor]:
+        env, env2 = self._env, self._env2
+        A = env.mps.tensors[site]
+        N = A.numel()
+        lib = _lib.lib()
+        stream = _lib.stream_ptr(self.device)
+        if site > 0:
+            if site not in env.left:
+                env.ensure_left(site)
+            if site not in env2.left:
+                env2.ensure_left(site)
+        l, Dl, Dr, d, r = env.site_dims(site)
+        _, D2l, D2r, _, _ = env2.site_dims(site)
+        vec = torch.empty(N, dtype=torch.float64, device=self.device)
+        hvec = torch.empty(N, dtype=torch.float64, device=self.device)
+        if N < self._exact_solver_dim:
+            # scipy.linalg.eigh(a, b) branch of linalg.eigh (tnpy/linalg.py:42-61)
+            Ma = env.full_matrix_device(site)
+            Mb = env2.full_matrix_device(site)
+            wp, wb = env.workspace.get(lib.tnx_sygv_lowest_workspace_bytes(N))
+            ev = C.c_double(0.0)
+            _lib.check(lib.tnx_sygv_lowest(_lib.ptr(Ma, "A"), _lib.ptr(Mb, "B"), N, _lib.ptr(vec, "x"), C.byref(ev),
+                                           wp, wb, stream), "tnx_sygv_lowest")
+            env.matvec_device(site, vec, hvec)
+            self.stats["dense_sites"] += 1
+            return float(ev.value), vec, hvec
+        opts = self._opts(tol, kwargs)
+        opts.precision = _lib.TNX_F64
+        info = _lib.EigshInfo()
+        wp, wb = env.workspace.get(lib.tnx_eigsh_gen_heff1_workspace_bytes(l, Dl, Dr, D2l, D2r, d, r, opts.max_basis))
+        st = lib.tnx_eigsh_gen_heff1(
+            _lib.ptr(env.left[site], "L"), _lib.ptr(env._W[site], "W"), _lib.ptr(env.right[site], "R"),
+            _lib.ptr(env2.left[site], "L2"), _lib.ptr(env2._W[site], "W2"), _lib.ptr(env2.right[site], "R2"),
+            _lib.ptr(A.reshape(-1), "v0"), _lib.ptr(vec, "vec"), _lib.ptr(hvec, "hvec"),
+            l, Dl, Dr, D2l, D2r, d, r, C.byref(opts), C.byref(info), wp, wb, stream,
+        )
+        if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
+            _lib.check(st, "tnx_eigsh_gen_heff1")
+        if st == _lib.TNX_ERR_NOCONV:
+            logger.warning(f"Generalized eigensolver at site {site} stopped after {info.matvecs} matvecs "
+                           f"with residual {info.resnorm:.3e}.")
+        self.stats["matvecs"] += info.matvecs
+        self.stats["matvecs_f64"] += info.matvecs_f64
+        self.stats["restarts"] += info.restarts
+        self.stats["krylov_sites"] += 1
+        return float(info.eval), vec, hvec
+
+    def _site_step(self, site: int, direction: Direction, tol: float, **kwargs) -> float:
+        energy, vec, hvec = self._solve(site, tol, **kwargs)
+        logger.info(f"Sweeping to site [{site + 1}/{self.n_sites}], E0 = {1 / energy + self._offset}")
+        # update_mps + perturb_wave_function with the (H - eps) operator
+        _lib.check(
+            _lib.lib().tnx_axpy(vec.numel(), 1e-5, _lib.ptr(hvec, "hvec"), _lib.ptr(vec, "vec"),
+                                _lib.stream_ptr(self.device)),
+            "tnx_axpy",
+        )
+        self._env.update_mps(site, vec)
+        self._env.split_tensor(site, direction)
+        self._env.update(site, direction)
+        self._env2.update(site, direction)
+        self.site_energies.append(energy)
+        if self.on_site is not None:
+            self.on_site(site, energy)
+        return energy
+
+    def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8, **kwargs) -> Optional[float]:
+        if direction == Direction.LEFTWARD:
+            self._env2.ensure_left(self.n_sites - 1)
+        return super().sweep(direction, tol=tol, **kwargs)
+
+    def _restore_mps(self) -> None:
+        """psi = (H - eps) phi as an MPS: each site's MPO tensor applied to
+        the state's, the bonds fused state-major (``:316-345``)."""
+        lib = _lib.lib()
+        stream = _lib.stream_ptr(self.device)
+        arrays = []
+        for site, (A, W) in enumerate(zip(self.mps.tensors, self._env._W)):
+            l, d, r = A.shape
+            Dl, Dr = int(W.shape[0]), int(W.shape[1])
+            out = torch.empty(l * Dl, d, r * Dr, dtype=torch.float64, device=self.device)
+            _lib.check(lib.tnx_mpo_apply_site(_lib.ptr(W, "W"), _lib.ptr(A, "A"), Dl, Dr, d, l, r,
+                                              _lib.ptr(out, "out"), stream), "tnx_mpo_apply_site")
+            arrays.append(out)
+        self._restored_mps = MatrixProductState(arrays, device=self.device)
+
+    def run(self, tol: float = 1e-7, max_sweep: int = 100, metric: Metric = Metric.ENERGY, **kwargs) -> List[float]:
+        """Sweeps as ``FiniteDMRG.run``; returns 1 / lambda + offset per sweep
+        and restores the eigenstate (``:362-372``)."""
+        energies = super().run(tol, max_sweep, metric, **kwargs)
+        self._restore_mps()
+        return (np.reciprocal(energies) + self._offset).tolist()
+
+    update = run
+
+    @property
+    def measurements(self) -> MatrixProductStateMeasurements:
+        if len(self._energies) == 1:
+            raise RuntimeError("FiniteDMRG is probably not executed yet.")
+        return MatrixProductStateMeasurements(self.restored_mps)
+
+    def variance(self) -> float:
+        """<psi|(H - eps)^2|psi> - lambda^2 - eps^2 on the restored state, the
+        reference's expression (``:385-391``)."""
+        if self._restored_mps is None:
+            raise RuntimeError("ShiftInvertDMRG is probably not executed yet.")
+        meas = MatrixProductStateMeasurements(self._restored_mps)
+        return meas.expectation_value(self._env.mpo.square()) - self._energies[-1] ** 2 - self._offset**2

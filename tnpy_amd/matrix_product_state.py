@@ -293,7 +293,8 @@ class MatrixProductState:
 
     def schmidt_values(self, normalize: bool = True) -> List[np.ndarray]:
         """Singular values across every bond (gauge invariant), from the
-        R factors of a left QR sweep of a right-canonicalised copy. With
+        R factors of a left QR sweep of a right-canonicalised copy (each R
+        through the library's SVD). With
         ``normalize`` the state is scaled to unit norm first (the DMRG state
         is not normalised, ``tnpy/finite_dmrg.py:139-141``)."""
         work = self.copy()
@@ -309,7 +310,7 @@ class MatrixProductState:
             R = torch.empty(r, r, dtype=torch.float64, device=self.device)
             wp, wb = ws.get(_lib.lib().tnx_qr_workspace_bytes(l * p, r))
             _lib.check(_lib.lib().tnx_qr(_lib.ptr(A, "A"), l * p, r, _lib.ptr(R, "R"), wp, wb, stream), "tnx_qr")
-            out.append(torch.linalg.svdvals(R).cpu().numpy())
+            out.append(_lib.svd_trunc(R, r, ws)[2])
             work._tensors[i + 1] = (R @ nxt.reshape(r, -1)).reshape(nxt.shape).contiguous()
         return out
 
