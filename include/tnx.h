@@ -96,6 +96,20 @@ int tnx_heff1_matvec(const double* L, const double* W, const double* R,
                      int d, int chir, int precision, void* ws, size_t ws_bytes,
                      tnx_stream_t stream);
 
+/* Allocation-free tnx_heff1_matvec. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * kernels and results (bitwise) as tnx_heff1_matvec, with the scratch in a
+ * fixed per-device array of tnx_noalloc_scratch_bytes() = 64 MiB instead of
+ * a workspace; sizes whose tnx_heff1_workspace_bytes exceeds it return
+ * TNX_ERR_INVALID. Calls no hipMalloc/hipFree. All calls on one GPU share
+ * the array: calls that overlap on one GPU race and must be serialised by
+ * the caller. */
+size_t tnx_noalloc_scratch_bytes(void);
+int tnx_heff1_matvec_noalloc(const double* L, const double* W, const double* R,
+                             const double* x, double* y, int chil, int Dl, int Dr,
+                             int d, int chir, int precision, tnx_stream_t stream);
+
 /* Dense H_eff, M[(l,k,r)][(l',b,r')]. Replaces
  * Environment.one_site_full_matrix (tnpy/matrix_product_state.py:372-409). */
 int tnx_heff1_dense(const double* L, const double* W, const double* R,

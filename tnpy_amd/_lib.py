@@ -67,6 +67,8 @@ SIGNATURES = {
     "tnx_gemm_f64": (_i, [_i, _i, _i, _i, _d, _p, _i64, _i64, _i, _p, _i64, _i64, _i, _d, _p, _i64, _i64, _i, _p]),
     "tnx_heff1_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "tnx_heff1_matvec": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_noalloc_scratch_bytes": (_sz, []),
+    "tnx_heff1_matvec_noalloc": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p]),
     "tnx_heff1_dense": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _p]),
     "tnx_env_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "tnx_env_left": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
