@@ -61,6 +61,15 @@ int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
                  double beta, double* C, int64_t ldc, int64_t strideC,
                  int precision, tnx_stream_t stream);
 
+/* The same GEMM with each operand's element type chosen (f32 flag: 1 =
+ * float, 0 = double). With precision TNX_F32 or TNX_BF16X3 and float
+ * operands the tiles are staged from FP32 directly (the values a double
+ * operand would be rounded to, half the bytes). */
+int tnx_gemm(int M, int N, int K, int batch, double alpha, const void* A, int a_f32,
+             int64_t lda, int64_t strideA, int a_kcontig, const void* B, int b_f32,
+             int64_t ldb, int64_t strideB, int b_ncontig, double beta, void* C, int c_f32,
+             int64_t ldc, int64_t strideC, int precision, tnx_stream_t stream);
+
 /* GEMM tile choice. Every GEMM takes its tile configuration from the tile
  * table (shape -> configuration) when the shape is there, else from a fixed
  * heuristic; neither allocates, synchronises or changes results (only
@@ -109,6 +118,15 @@ size_t tnx_noalloc_scratch_bytes(void);
 int tnx_heff1_matvec_noalloc(const double* L, const double* W, const double* R,
                              const double* x, double* y, int chil, int Dl, int Dr,
                              int d, int chir, int precision, tnx_stream_t stream);
+
+/* The one-site matvec with FP32 environments (env_f32: L and R are float
+ * copies, the form the low-precision cycles of tnx_eigsh_heff1 use; needs
+ * precision TNX_F32 or TNX_BF16X3) and/or FP32 input (x_f32). tnx_to_f32
+ * rounds n doubles to floats (device). */
+int tnx_heff1_matvec_ex(const void* L, const double* W, const void* R, const void* x, double* y,
+                        int chil, int Dl, int Dr, int d, int chir, int env_f32, int x_f32,
+                        int precision, void* ws, size_t ws_bytes, tnx_stream_t stream);
+int tnx_to_f32(const double* x, float* y, int64_t n, tnx_stream_t stream);
 
 /* Dense H_eff, M[(l,k,r)][(l',b,r')]. Replaces
  * Environment.one_site_full_matrix (tnpy/matrix_product_state.py:372-409). */

@@ -65,6 +65,10 @@ SIGNATURES = {
     "tnx_tune_load": (_i, [C.c_char_p]),
     "tnx_tune_save": (_i, [C.c_char_p]),
     "tnx_gemm_f64": (_i, [_i, _i, _i, _i, _d, _p, _i64, _i64, _i, _p, _i64, _i64, _i, _d, _p, _i64, _i64, _i, _p]),
+    "tnx_gemm": (_i, [_i, _i, _i, _i, _d, _p, _i, _i64, _i64, _i, _p, _i, _i64, _i64, _i, _d, _p, _i, _i64, _i64, _i,
+                      _p]),
+    "tnx_heff1_matvec_ex": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
+    "tnx_to_f32": (_i, [_p, _p, _i64, _p]),
     "tnx_heff1_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "tnx_heff1_matvec": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _sz, _p]),
     "tnx_noalloc_scratch_bytes": (_sz, []),

@@ -102,6 +102,12 @@ static int launch_wapply(const WApply& a, hipStream_t s) {
   return TNX_OK;
 }
 
+__global__ void to_f32_cvt_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
 static bool valid_dims(int chil, int Dl, int Dr, int d, int chir) {
   return chil > 0 && Dl > 0 && Dr > 0 && d > 0 && chir > 0;
 }
@@ -130,16 +136,17 @@ size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir) {
 // contiguous stacks; x_p = x + p*sx, y_p = y + p*sy (sx = sy = chil*d*chir
 // for contiguous stacks). One problem: batch = count = 1. x is FP64, or
 // FP32 when x_f32 (a low-precision Krylov basis); y is FP64.
-int heff1_matvec_b(const double* L, const double* W, const double* R, const void* x, int x_f32,
+int heff1_matvec_b(const void* L, const double* W, const void* R, const void* x, int x_f32,
                    double* y, int chil, int Dl, int Dr, int d, int chir, int prec, int batch,
                    int count, const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
-                   hipStream_t s) {
+                   hipStream_t s, int env_f32) {
   TNX_CHECK_ARG(valid_dims(chil, Dl, Dr, d, chir), "heff1_matvec: bad dims");
   TNX_CHECK_ARG(batch >= 1 && batch <= 65535, "heff1_matvec: bad batch %d", batch);
   TNX_CHECK_ARG(count >= 1 && count <= batch && (bidx || count == batch),
                 "heff1_matvec: bad count %d", count);
   TNX_CHECK_ARG(L && W && R && x && y, "heff1_matvec: null pointer");
   TNX_CHECK_ARG(prec >= TNX_F64 && prec <= TNX_BF16X3, "heff1_matvec: bad precision");
+  TNX_CHECK_ARG(!env_f32 || prec != TNX_F64, "heff1_matvec: FP32 environments need a low-precision arithmetic");
   TNX_CHECK_ARG(ws && ws_bytes >= heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch),
                 "heff1_matvec: workspace too small");
   const size_t eb = inter_bytes(prec);
@@ -153,7 +160,7 @@ int heff1_matvec_b(const double* L, const double* W, const double* R, const void
   GemmDesc g1{};
   g1.M = chil * d; g1.N = Dr * chir; g1.K = chir; g1.batch = count; g1.bidx = bidx;
   g1.A = x; g1.ta = x_f32 ? DT::F32 : DT::F64; g1.lda = chir; g1.sA = sx; g1.a_kcontig = true;
-  g1.B = R; g1.tb = DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = (int64_t)chir * Dr * chir; g1.b_ncontig = true;
+  g1.B = R; g1.tb = env_f32 ? DT::F32 : DT::F64; g1.ldb = (int64_t)Dr * chir; g1.sB = (int64_t)chir * Dr * chir; g1.b_ncontig = true;
   g1.C = T1; g1.tc = it; g1.ldc = (int64_t)Dr * chir; g1.sC = nt1;
   g1.alpha = 1.0; g1.beta = 0.0; g1.prec = (Prec)prec;
   TNX_TRY(gemm(g1, s));
@@ -170,7 +177,7 @@ int heff1_matvec_b(const double* L, const double* W, const double* R, const void
   // y[b, (c,s)] = sum_{(a,w)} L[(a,w), b] T2[(a,w), (c,s)]
   GemmDesc g3{};
   g3.M = chil; g3.N = d * chir; g3.K = chil * Dl; g3.batch = count; g3.bidx = bidx;
-  g3.A = L; g3.ta = DT::F64; g3.lda = chil; g3.sA = (int64_t)chil * Dl * chil; g3.a_kcontig = false;
+  g3.A = L; g3.ta = env_f32 ? DT::F32 : DT::F64; g3.lda = chil; g3.sA = (int64_t)chil * Dl * chil; g3.a_kcontig = false;
   g3.B = T2; g3.tb = it; g3.ldb = (int64_t)d * chir; g3.sB = nt2; g3.b_ncontig = true;
   g3.C = y; g3.tc = DT::F64; g3.ldc = (int64_t)d * chir; g3.sC = sy;
   g3.alpha = 1.0; g3.beta = 0.0; g3.prec = (Prec)prec;
@@ -184,7 +191,7 @@ int heff1_matvec(const double* L, const double* W, const double* R,
                  int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s) {
   const int64_t nx = (int64_t)chil * d * chir;
   return heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, prec, 1, 1, nullptr, nx, nx, ws,
-                        ws_bytes, s);
+                        ws_bytes, s, 0);
 }
 
 // Two-site effective Hamiltonian (no reference function: the two-site
@@ -510,13 +517,30 @@ size_t tnx_heff1_batched_workspace_bytes(int chil, int Dl, int Dr, int d, int ch
   return tnx::heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch);
 }
 
+int tnx_heff1_matvec_ex(const void* L, const double* W, const void* R, const void* x, double* y,
+                        int chil, int Dl, int Dr, int d, int chir, int env_f32, int x_f32,
+                        int precision, void* ws, size_t ws_bytes, tnx_stream_t stream) {
+  const int64_t nx = (int64_t)chil * d * chir;
+  return tnx::heff1_matvec_b(L, W, R, x, x_f32, y, chil, Dl, Dr, d, chir, precision, 1, 1, nullptr,
+                             nx, nx, ws, ws_bytes, (hipStream_t)stream, env_f32);
+}
+
+int tnx_to_f32(const double* x, float* y, int64_t n, tnx_stream_t stream) {
+  TNX_CHECK_ARG(x && y && n >= 0, "tnx_to_f32: bad arguments");
+  if (n == 0) return TNX_OK;
+  const int nb = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(tnx::to_f32_cvt_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, x, y, n);
+  TNX_LAUNCH_CHECK();
+  return TNX_OK;
+}
+
 int tnx_heff1_matvec_batched(const double* L, const double* W, const double* R,
                              const double* x, double* y, int chil, int Dl, int Dr,
                              int d, int chir, int batch, int precision, void* ws,
                              size_t ws_bytes, tnx_stream_t stream) {
   const int64_t nx = (int64_t)chil * d * chir;
   return tnx::heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, precision, batch, batch,
-                             nullptr, nx, nx, ws, ws_bytes, (hipStream_t)stream);
+                             nullptr, nx, nx, ws, ws_bytes, (hipStream_t)stream, 0);
 }
 
 int tnx_heff1_dense_batched(const double* L, const double* W, const double* R,

@@ -75,7 +75,8 @@ bool gemm_typed_ok(const GemmDesc& d, int kc) {
   const bool f64f64 = d.ta == DT::F64 && d.tb == DT::F64;
   const bool f64f32 = d.ta == DT::F64 && d.tb == DT::F32;
   const bool f32f64 = d.ta == DT::F32 && d.tb == DT::F64;
-  if (!(f64f64 || f64f32 || (d.prec != Prec::F64 && f32f64))) return false;
+  const bool f32f32 = d.ta == DT::F32 && d.tb == DT::F32;
+  if (!(f64f64 || f64f32 || (d.prec != Prec::F64 && (f32f64 || f32f32)))) return false;
   const int nb2 = d.nb2 > 0 ? d.nb2 : 1;
   // the K range of a split is a whole number of vectors when K-contiguous
   if (d.a_kcontig && kc % (d.ta == DT::F64 ? 2 : 4) != 0) return false;
@@ -90,15 +91,18 @@ bool gemm_typed_ok(const GemmDesc& d, int kc) {
 }
 
 // Deterministic split-K rule: split only when the output is small (fewer
-// than ~512 tiles of 128x128) and K is long, and the scratch fits.
+// than ~512 tiles of 128x128), each split keeps at least 64 of K (two k
+// steps of the 64x64 tile), and the scratch fits. Skinny outputs with a
+// long K (the 32-row V^T A products of the blocked QR) otherwise run on a
+// few workgroups, each looping over all of K.
 static int split_rule(const GemmDesc& d) {
   if (!d.splitk_ws) return 1;
   const int64_t elems = (int64_t)d.M * d.N * d.batch;
   // about two 256x128 output tiles per CU after splitting
   const int64_t target = (int64_t)2 * 256 * 128 * 128;
-  if (elems >= target || d.K < 1024) return 1;
+  if (elems >= target) return 1;
   int sp = 1;
-  while (sp < 32 && elems * sp * 2 <= target && (int64_t)d.K / (sp * 2) >= 256) sp *= 2;
+  while (sp < 32 && elems * sp * 2 <= target && (int64_t)d.K / (sp * 2) >= 64) sp *= 2;
   while (sp > 1 && (size_t)sp * elems * sizeof(double) > d.splitk_ws_bytes) sp /= 2;
   return sp;
 }
@@ -372,6 +376,22 @@ extern "C" int tnx_gemm_f64(int M, int N, int K, int batch, double alpha,
   d.A = A; d.ta = DT::F64; d.lda = lda; d.sA = strideA; d.a_kcontig = a_kcontig != 0;
   d.B = B; d.tb = DT::F64; d.ldb = ldb; d.sB = strideB; d.b_ncontig = b_ncontig != 0;
   d.C = C; d.tc = DT::F64; d.ldc = ldc; d.sC = strideC;
+  d.alpha = alpha; d.beta = beta; d.prec = (Prec)precision;
+  return gemm(d, (hipStream_t)stream);
+}
+
+extern "C" int tnx_gemm(int M, int N, int K, int batch, double alpha, const void* A, int a_f32,
+                        int64_t lda, int64_t strideA, int a_kcontig, const void* B, int b_f32,
+                        int64_t ldb, int64_t strideB, int b_ncontig, double beta, void* C,
+                        int c_f32, int64_t ldc, int64_t strideC, int precision,
+                        tnx_stream_t stream) {
+  using namespace tnx;
+  TNX_CHECK_ARG(precision >= TNX_F64 && precision <= TNX_BF16X3, "tnx_gemm: bad precision %d", precision);
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K; d.batch = batch;
+  d.A = A; d.ta = a_f32 ? DT::F32 : DT::F64; d.lda = lda; d.sA = strideA; d.a_kcontig = a_kcontig != 0;
+  d.B = B; d.tb = b_f32 ? DT::F32 : DT::F64; d.ldb = ldb; d.sB = strideB; d.b_ncontig = b_ncontig != 0;
+  d.C = C; d.tc = c_f32 ? DT::F32 : DT::F64; d.ldc = ldc; d.sC = strideC;
   d.alpha = alpha; d.beta = beta; d.prec = (Prec)precision;
   return gemm(d, (hipStream_t)stream);
 }

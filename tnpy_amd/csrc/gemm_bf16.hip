@@ -8,7 +8,8 @@ void gemm_launch_bf16(int ci, const GemmDesc& d, const GemmArgs& g, int splits, 
   if (!typed) return gemm_launch_index<2, void, void>(ci, d, g, splits, s);
   if (d.ta == DT::F64 && d.tb == DT::F64) return gemm_launch_index<2, double, double>(ci, d, g, splits, s);
   if (d.ta == DT::F64) return gemm_launch_index<2, double, float>(ci, d, g, splits, s);
-  return gemm_launch_index<2, float, double>(ci, d, g, splits, s);
+  if (d.tb == DT::F64) return gemm_launch_index<2, float, double>(ci, d, g, splits, s);
+  return gemm_launch_index<2, float, float>(ci, d, g, splits, s);
 }
 
 }  // namespace tnx

@@ -37,10 +37,10 @@ size_t heff1_ws_bytes(int chil, int Dl, int Dr, int d, int chir);
 int heff2_matvec(const double* L, const double* W1, const double* W2, const double* R,
                  const void* x, int x_f32, double* y, int chil, int Dl, int Dm, int Dr, int d,
                  int chir, int prec, void* ws, size_t ws_bytes, hipStream_t s);
-int heff1_matvec_b(const double* L, const double* W, const double* R, const void* x, int x_f32,
+int heff1_matvec_b(const void* L, const double* W, const void* R, const void* x, int x_f32,
                    double* y, int chil, int Dl, int Dr, int d, int chir, int prec, int batch,
                    int count, const int* bidx, int64_t sx, int64_t sy, void* ws, size_t ws_bytes,
-                   hipStream_t s);
+                   hipStream_t s, int env_f32);
 size_t heff2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir);
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1109,14 +1109,19 @@ static size_t gd_ws_bytes(size_t N, int m) {
   return align256((size_t)m * N * 8) + align256((size_t)8 * (kMaxBasis + 8) * 8);
 }
 
-static size_t eigsh_single_ws_bytes(size_t N, size_t mvb, int max_basis) {
+static size_t eigsh_single_ws_bytes(size_t N, size_t mvb, int max_basis, size_t env32_bytes = 0) {
   const int m = std::max(3, std::min(max_basis, kMaxBasis));
-  return eigsh_ws_bytes_n(N, mvb, max_basis) + gd_ws_bytes(N, m);
+  return eigsh_ws_bytes_n(N, mvb, max_basis) + gd_ws_bytes(N, m) + align256(env32_bytes);
+}
+
+// FP32 copies of L and R for the low-precision cycles of a one-site solve
+static size_t env32_bytes_of(int chil, int Dl, int Dr, int chir) {
+  return align256((size_t)chil * Dl * chil * 4) + align256((size_t)chir * Dr * chir * 4);
 }
 
 size_t eigsh_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis) {
   return eigsh_single_ws_bytes((size_t)chil * d * chir, heff1_ws_bytes(chil, Dl, Dr, d, chir),
-                               max_basis);
+                               max_basis, env32_bytes_of(chil, Dl, Dr, chir));
 }
 
 size_t eigsh2_ws_bytes(int chil, int Dl, int Dm, int Dr, int d, int chir, int max_basis) {
@@ -1299,6 +1304,15 @@ struct EigCtx {
   int matvecs = 0, matvecs64 = 0;
   double* Wd = nullptr;   // FP64 correction: H V (m x N)
   double* gsm = nullptr;  // its small arrays (8 x (kMaxBasis + 8))
+  // one-site problems: L and R rounded to FP32 at most once per solve (at
+  // its first low-precision cycle), read by the low-precision matvecs (the
+  // same values their tile staging would round to, half the bytes, no
+  // conversion per matvec); null until converted and for FP64 solves
+  const float* L32 = nullptr;
+  const float* R32 = nullptr;
+  float* L32_buf = nullptr;  // their workspace (set for low-precision solves)
+  float* R32_buf = nullptr;
+  size_t env32_bytes = 0;  // workspace reserved for them
 };
 
 // Pinned host blocks for the per-step status reads of the Lanczos solvers: a
@@ -1378,8 +1392,11 @@ static int apply_h(EigCtx& c, const void* x, int x_f32, double* y, int prec) {
   if (c.W2)
     return heff2_matvec(c.L, c.W, c.W2, c.R, x, x_f32, y, c.chil, c.Dl, c.Dm, c.Dr, c.d, c.chir,
                         prec, c.k.mv, c.k.mv_bytes, c.s);
+  if (prec != TNX_F64 && c.L32)
+    return heff1_matvec_b(c.L32, c.W, c.R32, x, x_f32, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, 1,
+                          1, nullptr, c.N, c.N, c.k.mv, c.k.mv_bytes, c.s, 1);
   return heff1_matvec_b(c.L, c.W, c.R, x, x_f32, y, c.chil, c.Dl, c.Dr, c.d, c.chir, prec, 1, 1,
-                        nullptr, c.N, c.N, c.k.mv, c.k.mv_bytes, c.s);
+                        nullptr, c.N, c.N, c.k.mv, c.k.mv_bytes, c.s, 0);
 }
 
 static int read_status(EigCtx& c, double* host, int n) {
@@ -1593,6 +1610,7 @@ int eigsh_heff1(const double* L, const double* W, const double* R, const double*
   c.chil = chil; c.Dl = Dl; c.Dr = Dr; c.d = d; c.chir = chir;
   c.N = (int64_t)chil * d * chir;
   c.s = s;
+  c.env32_bytes = env32_bytes_of(chil, Dl, Dr, chir);
   return eigsh_run(c, v0, vec, hvec, opts, info, ws, ws_bytes, heff1_ws_bytes(chil, Dl, Dr, d, chir));
 }
 
@@ -1618,7 +1636,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
   TNX_CHECK_ARG(opts->precision >= TNX_F64 && opts->precision <= TNX_BF16X3, "eigsh: bad precision");
   TNX_CHECK_ARG(opts->tol >= 0.0, "eigsh: negative tol");
   const int m = std::max(3, std::min(opts->max_basis, kMaxBasis));
-  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_single_ws_bytes((size_t)c.N, mvb, m),
+  TNX_CHECK_ARG(ws && ws_bytes >= eigsh_single_ws_bytes((size_t)c.N, mvb, m, c.env32_bytes),
                 "eigsh: workspace too small");
   hipStream_t s = c.s;
   carve(c.k, ws, m, (size_t)c.N, mvb);
@@ -1628,7 +1646,24 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
                (eigsh_ws_bytes_n((size_t)c.N, mvb, m) - 256);
     c.Wd = (double*)gp;
     c.gsm = (double*)(gp + align256((size_t)m * c.N * 8));
+    if (c.env32_bytes && !c.W2 && opts->precision != TNX_F64) {
+      // converted at the first low-precision cycle (a solve that converges
+      // on its first FP64 step never reads them)
+      c.L32_buf = (float*)(gp + gd_ws_bytes((size_t)c.N, m));
+      c.R32_buf = (float*)((char*)c.L32_buf + align256((size_t)c.chil * c.Dl * c.chil * 4));
+    }
   }
+  auto env_to_f32 = [&]() -> int {
+    if (c.L32 || !c.L32_buf) return TNX_OK;
+    const int64_t nl = (int64_t)c.chil * c.Dl * c.chil, nr = (int64_t)c.chir * c.Dr * c.chir;
+    hipLaunchKernelGGL(to_f32_kernel, dim3(vec_blocks(nl)), dim3(256), 0, s, c.L, c.L32_buf, nl);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(to_f32_kernel, dim3(vec_blocks(nr)), dim3(256), 0, s, c.R, c.R32_buf, nr);
+    TNX_LAUNCH_CHECK();
+    c.L32 = c.L32_buf;
+    c.R32 = c.R32_buf;
+    return TNX_OK;
+  };
   KWork& k = c.k;
   const int64_t N = c.N;
   const int prec = opts->precision;
@@ -1721,12 +1756,17 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       // V[0..jj] (sums in h1, h2, h3).
       auto front = [&](int jj) -> int {
         if constexpr (f32) {
-          // the matvec reads FP64 input faster than FP32 (16-byte vs 8-byte
-          // tile loads): widen V[jj] into k.y, which is free until the
-          // cycle's Ritz vector is formed
-          hipLaunchKernelGGL(to_f64_kernel, dim3(nb), dim3(256), 0, s, V + (int64_t)jj * N, k.y, N);
-          TNX_LAUNCH_CHECK();
-          TNX_TRY(apply_h(c, k.y, 0, k.w, cprec));
+          if (c.L32) {
+            // FP32 basis row and FP32 environments: the matvec's GEMMs stage
+            // both operands with 16-byte loads of floats
+            TNX_TRY(apply_h(c, V + (int64_t)jj * N, 1, k.w, cprec));
+          } else {
+            // FP64 environments: widen V[jj] into k.y (free until the
+            // cycle's Ritz vector is formed) for 16-byte FP64 tile loads
+            hipLaunchKernelGGL(to_f64_kernel, dim3(nb), dim3(256), 0, s, V + (int64_t)jj * N, k.y, N);
+            TNX_LAUNCH_CHECK();
+            TNX_TRY(apply_h(c, k.y, 0, k.w, cprec));
+          }
         } else {
           TNX_TRY(apply_h(c, V + (int64_t)jj * N, 0, k.w, cprec));
         }
@@ -1842,6 +1882,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
       }
     };
     if (mixed && N % 2 == 0) {
+      TNX_TRY(env_to_f32());
       TNX_TRY(cycle(reinterpret_cast<float*>(k.V), reinterpret_cast<float*>(k.Y)));
     } else {
       TNX_TRY(cycle(k.V, k.Y));
@@ -1971,7 +2012,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
   auto matvec = [&](const double* x, double* y, int cnt) -> int {
     matvecs++;
     return heff1_matvec_b(L, W, R, x, 0, y, chil, Dl, Dr, d, chir, TNX_F64, batch, cnt, idx, sp,
-                          sp, mv, mvb, s);
+                          sp, mv, mvb, s, 0);
   };
 
   TNX_TRY(upload(idx, act));

@@ -356,7 +356,9 @@ __device__ void lds_form_q(double* S, int ld, int m, int n, const double* tau_s)
 __global__ __launch_bounds__(256) void tsqr_leaf_kernel(const double* __restrict__ A, int n, int jb,
                                                        int mp, int b, int h, int nleaf,
                                                        double* __restrict__ Rst,
-                                                       double* __restrict__ Qst) {
+                                                       double* __restrict__ Qst,
+                                                       const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;  // panel already factored by the Cholesky-QR path
   __shared__ double S[kTsMaxRows * kTsLd];
   __shared__ double tau_s[kTsNB];
   __shared__ double red[4];
@@ -393,7 +395,17 @@ __global__ __launch_bounds__(512) void tsqr_root_kernel(const double* __restrict
                                                        double* __restrict__ Vall,
                                                        double* __restrict__ Tst,
                                                        double* __restrict__ Qtop,
-                                                       double* __restrict__ Ust) {
+                                                       double* __restrict__ Ust,
+                                                       const int* __restrict__ flag,
+                                                       const double* __restrict__ Rbuf) {
+  if (flag && *flag == 0) {
+    // the Cholesky-QR factorisation was accepted: its R (S R, b x b) into A
+    for (int e = threadIdx.x; e < b * b; e += 512) {
+      const int i = e / b, c = e - i * b;
+      if (c >= i) A[(int64_t)(jb + i) * n + jb + c] = Rbuf[i * kTsNB + c];
+    }
+    return;
+  }
   __shared__ double S[kTsMaxLeaves * kTsNB * kTsLd];
   __shared__ double Rb[kTsNB * kTsLd];
   __shared__ double B[kTsNB * kTsLd];
@@ -492,7 +504,9 @@ __global__ __launch_bounds__(256) void tsqr_apply_kernel(const double* __restric
                                                         int b, int h, int nleaf, int n, int jb,
                                                         double* __restrict__ Vall,
                                                         double* __restrict__ Y1,
-                                                        double* __restrict__ Y2) {
+                                                        double* __restrict__ Y2,
+                                                        const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
   __shared__ double M[kTsNB * kTsNB];
   __shared__ double U[kTsNB * kTsNB];
   __shared__ double T[kTsNB * kTsNB];
@@ -550,6 +564,472 @@ __global__ __launch_bounds__(256) void tsqr_apply_kernel(const double* __restric
   }
 }
 
+// ---- Cholesky-QR panels ----------------------------------------------------------
+// The panel P (mp x b, b <= kTsNB) is first factored by CholeskyQR2
+// (Fukaya, Nakatsukasa, Yanagisawa, Yamamoto, "CholeskyQR2: a simple and
+// communication-avoiding algorithm for computing a tall-skinny QR
+// factorization", 2014): R1^T R1 = P^T P, Q1 = P R1^-1, R2^T R2 = Q1^T Q1,
+// Q = Q1 R2^-1, R = R2 R1; Q is orthonormal and P - Q R at rounding level
+// for cond(P) below about 1e7. Three kernels of many workgroups (kCqRB panel
+// rows each) per panel; in the first two each workgroup writes the Gram
+// partial of its rows, and the workgroup that arrives last (atomic ticket
+// after a release fence) sums the partials in a fixed order and does the
+// b x b work once, in one wave with a row per lane (no barriers in the
+// serial steps): the Cholesky factor and its inverse, and after the second
+// pass R and the Householder reconstruction of the panel from the explicit
+// Q as the TSQR root does (LU of Q1 - S). The third kernel forms the rows
+// of Q, V, V T^T and V T.
+// The factorisation is accepted when both Cholesky pivots are positive and
+// finite, the second Gram is within 0.1 of the identity, and every row of
+// P - Q R is within 2e-14 ||P||_F (CholeskyQR keeps Q orthonormal for
+// panels with exactly dependent columns but can lose R's accuracy there;
+// accepted panels measure below 1e-15 ||P||_F). Otherwise the panel's flag
+// is set and the TSQR kernels factor it: they return at once for a panel
+// whose flag is clear, the root then only copying the accepted R into A.
+constexpr int kCqRB = 64;         // panel rows per workgroup
+constexpr int kCqLd = kTsNB + 1;  // LDS row stride
+constexpr int kCqG = kTsNB * kTsNB;
+
+// Small per-QR buffers of the Cholesky-QR panels (device memory).
+struct CqBuf {
+  double* Qb;    // mp x kTsNB: the panel's rows after the first pass (Q1)
+  double* Gp;    // nblk x kCqG Gram partials (lower triangles)
+  double* Li;    // the current pass's L^-1
+  double* L1s;   // Cholesky factor of the first pass (R1 = L1^T)
+  double* Rs;    // R = R2 R1 (upper)
+  double* Rbuf;  // S R, the Householder-form R the TSQR root copies into A
+  double* L1f;   // unit lower factor of the LU of Q1 - S
+  double* Ui;    // U^-1
+  double* T;     // compact-WY T
+  double* pnorm; // ||P||_F
+  int* cnt;      // arrival ticket, zero between kernels
+};
+
+// Lane i's value of a wave-distributed double as a wave-uniform scalar
+// (v_readlane); i must be wave-uniform.
+__device__ __forceinline__ double qr_lane_val(double v, int i) {
+  const long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), i);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(u >> 32), i);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// True in the workgroup that finishes last; its reads then see every other
+// workgroup's writes made before their arrival.
+__device__ bool cq_arrive_last(int* cnt) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(cnt, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+// G = sum of nblk partial Grams, lower triangle, fixed order per entry, 16
+// loads in flight per thread; entries outside b x b are the identity's.
+__device__ void cq_gram_reduce(const double* __restrict__ Gp, int nblk, int b, double* G) {
+  int e[4];
+  bool on[4];
+  double s[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    e[q] = threadIdx.x + 256 * q;
+    const int i = e[q] >> 5, c = e[q] & 31;
+    on[q] = c <= i && i < b;
+    s[q] = 0.0;
+  }
+  for (int k = 0; k < nblk; k += 4) {
+    double v[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        v[u][q] = on[q] && k + u < nblk ? Gp[(size_t)(k + u) * kCqG + e[q]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s[q] += v[u][q];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = e[q] >> 5, c = e[q] & 31;
+    G[i * kCqLd + c] = on[q] ? s[q] : (i == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+}
+
+// Cholesky G = L L^T (lower triangle of G in LDS) by the calling wave: row i
+// in lane i's registers (lanes >= kTsNB mirror row kTsNB - 1), column j's
+// entries broadcast with v_readlane. L (zeros above the diagonal) to Lo,
+// 1/L_jj to rd. Returns false when a pivot is not positive and finite (the
+// same in every lane).
+__device__ bool wv_cholesky(const double* G, double* Lo, double* rd) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane < kTsNB ? lane : kTsNB - 1;
+  double g[kTsNB];
+#pragma unroll
+  for (int c = 0; c < kTsNB; ++c) g[c] = c <= i ? G[i * kCqLd + c] : 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < kTsNB; ++j) {
+    const double dj = qr_lane_val(g[j], j);
+    ok = ok && dj > 0.0 && dj <= 1.7976931348623157e308;
+    const double sq = sqrt(dj);
+    const double rs = 1.0 / sq;
+    const double l = i > j ? g[j] * rs : (i == j ? sq : 0.0);
+    g[j] = l;
+#pragma unroll
+    for (int c = j + 1; c < kTsNB; ++c) g[c] = fma(-l, qr_lane_val(l, c), g[c]);
+    if (lane == 0) rd[j] = rs;
+  }
+  if (lane < kTsNB) {
+#pragma unroll
+    for (int c = 0; c < kTsNB; ++c) Lo[i * kCqLd + c] = c <= i ? g[c] : 0.0;
+  }
+  return ok;
+}
+
+// LU of B - S without pivoting (B in LDS), S_jj = -sign of the pivot so
+// |U_jj| >= 1, by the calling wave with row i in lane i: the unit lower
+// factor to Lf, U to U (zeros elsewhere), S to sg, 1/U_jj to rdU.
+__device__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double* rdU) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane < kTsNB ? lane : kTsNB - 1;
+  double g[kTsNB];
+#pragma unroll
+  for (int c = 0; c < kTsNB; ++c) g[c] = B[i * kCqLd + c];
+#pragma unroll
+  for (int j = 0; j < kTsNB; ++j) {
+    const double a = qr_lane_val(g[j], j);
+    const double sgn = a >= 0.0 ? -1.0 : 1.0;
+    const double piv = a - sgn;
+    const double rp = 1.0 / piv;
+    const double l = g[j] * rp;
+#pragma unroll
+    for (int c = j + 1; c < kTsNB; ++c) {
+      const double u = qr_lane_val(g[c], j);  // U[j][c]: row j is final
+      if (i > j) g[c] = fma(-l, u, g[c]);
+    }
+    if (i > j) g[j] = l;
+    if (i == j) g[j] = piv;
+    if (lane == 0) { sg[j] = sgn; rdU[j] = rp; }
+  }
+  if (lane < kTsNB) {
+#pragma unroll
+    for (int c = 0; c < kTsNB; ++c) {
+      Lf[i * kCqLd + c] = c < i ? g[c] : (c == i ? 1.0 : 0.0);
+      U[i * kCqLd + c] = c >= i ? g[c] : 0.0;
+    }
+  }
+}
+
+// Li = L^-1 for lower triangular L (LDS; 1/L_ii in rd, or unit diagonal
+// when rd is null): column c in lane c of the calling wave, four
+// accumulation chains per entry. Zeros above the diagonal.
+__device__ void wv_lower_inverse(const double* L, const double* rd, double* Li) {
+  const int c = threadIdx.x & 63;
+  double x[kTsNB];
+#pragma unroll
+  for (int r = 0; r < kTsNB; ++r) {
+    double a0 = r == c ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int k = 0; k + 3 < r; k += 4) {
+      a0 = fma(-L[r * kCqLd + k], x[k], a0);
+      a1 = fma(-L[r * kCqLd + k + 1], x[k + 1], a1);
+      a2 = fma(-L[r * kCqLd + k + 2], x[k + 2], a2);
+      a3 = fma(-L[r * kCqLd + k + 3], x[k + 3], a3);
+    }
+#pragma unroll
+    for (int k = r & ~3; k < r; ++k) a0 = fma(-L[r * kCqLd + k], x[k], a0);
+    const double a = (a0 + a1) + (a2 + a3);
+    x[r] = rd ? a * rd[r] : a;
+  }
+  if (c < kTsNB) {
+#pragma unroll
+    for (int r = 0; r < kTsNB; ++r) Li[r * kCqLd + c] = x[r];
+  }
+}
+
+// Ui = U^-1 for upper triangular U (LDS; 1/U_ii in rd): column c in lane c
+// of the calling wave. Zeros below the diagonal.
+__device__ void wv_upper_inverse(const double* U, const double* rd, double* Ui) {
+  const int c = threadIdx.x & 63;
+  double x[kTsNB];
+#pragma unroll
+  for (int r = kTsNB - 1; r >= 0; --r) {
+    double a0 = r == c ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int k = r + 1; k + 3 < kTsNB; k += 4) {
+      a0 = fma(-U[r * kCqLd + k], x[k], a0);
+      a1 = fma(-U[r * kCqLd + k + 1], x[k + 1], a1);
+      a2 = fma(-U[r * kCqLd + k + 2], x[k + 2], a2);
+      a3 = fma(-U[r * kCqLd + k + 3], x[k + 3], a3);
+    }
+#pragma unroll
+    for (int k = kTsNB - ((kTsNB - 1 - r) & 3); k < kTsNB; ++k) a0 = fma(-U[r * kCqLd + k], x[k], a0);
+    x[r] = ((a0 + a1) + (a2 + a3)) * rd[r];
+  }
+  if (c < kTsNB) {
+#pragma unroll
+    for (int r = 0; r < kTsNB; ++r) Ui[r * kCqLd + c] = x[r];
+  }
+}
+
+// Y[row] = X[row] Li^T for nrows rows (Y = X R^-1 with R = L^T), four
+// threads per row, eight columns each; columns >= b are written as zero.
+__device__ void cq_rows_transform(const double* X, const double* Li, double* Y, int nrows, int b) {
+  const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
+  if (row >= nrows) return;
+  double y[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) y[t] = 0.0;
+  for (int k = 0; k < b; ++k) {
+    const double xv = X[row * kCqLd + k];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) y[t] += xv * Li[(part + 4 * t) * kCqLd + k];
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int c = part + 4 * t;
+    Y[row * kCqLd + c] = c < b ? y[t] : 0.0;
+  }
+}
+
+// Partial Gram Y^T Y (lower triangle) of the kCqRB rows in LDS (rows past
+// the panel are zero).
+__device__ void cq_gram_partial(const double* Y, double* __restrict__ Gout) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    const int i = e >> 5, c = e & 31;
+    if (c > i) continue;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+    for (int r = 0; r < kCqRB; r += 2) {
+      s0 += Y[r * kCqLd + i] * Y[r * kCqLd + c];
+      s1 += Y[(r + 1) * kCqLd + i] * Y[(r + 1) * kCqLd + c];
+    }
+    Gout[(size_t)blockIdx.x * kCqG + e] = s0 + s1;
+  }
+}
+
+__device__ void cq_store(const double* S, double* __restrict__ g) {
+  for (int e = threadIdx.x; e < kCqG; e += 256) g[e] = S[(e >> 5) * kCqLd + (e & 31)];
+}
+
+__device__ void cq_load(const double* __restrict__ g, double* S) {
+  for (int e = threadIdx.x; e < kCqG; e += 256) S[(e >> 5) * kCqLd + (e & 31)] = g[e];
+}
+
+__global__ void cq_init_kernel(int* cnt) { *cnt = 0; }
+
+// PASS 0: Gram of P; the last workgroup factors it (L1, L1^-1) and sets the
+// flag's first value. PASS 1: Q1 = P R1^-1 to Qb and its Gram; the last
+// workgroup checks it against the identity, factors it (L2, L2^-1), forms
+// R = (L1 L2)^T and rebuilds the Householder form from the top b rows of
+// Q = Q1 R2^-1: LU of Q1 - S (S_ii = -sign of the pivot, |U_ii| >= 1),
+// V = [L1; Q2' U^-1], T = -U S L1^-T.
+template <int PASS>
+__global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__ A, int n, int jb,
+                                                     int mp, int b, CqBuf q, int nblk,
+                                                     int* __restrict__ flag) {
+  __shared__ double X[kCqRB * kCqLd];
+  __shared__ double Y[kCqRB * kCqLd];
+  __shared__ double G[kTsNB * kCqLd];
+  __shared__ double Lo[kTsNB * kCqLd];
+  __shared__ double Li[kTsNB * kCqLd];
+  __shared__ double rd[kTsNB];
+  __shared__ int okf;
+  const int tid = threadIdx.x;
+  if (PASS > 0 && *flag) return;
+  const int r0 = blockIdx.x * kCqRB;
+  if (PASS > 0) cq_load(q.Li, Li);
+  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+    const int i = e >> 5, c = e & 31;
+    X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+  }
+  __syncthreads();
+  const double* Yr = X;
+  if (PASS > 0) {
+    cq_rows_transform(X, Li, Y, kCqRB, b);
+    __syncthreads();
+    for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+      const int i = e >> 5, c = e & 31;
+      if (r0 + i < mp) q.Qb[(size_t)(r0 + i) * kTsNB + c] = Y[i * kCqLd + c];
+    }
+    Yr = Y;
+  }
+  cq_gram_partial(Yr, q.Gp);
+  if (!cq_arrive_last(q.cnt)) return;
+  if (tid == 0) okf = 1;
+  cq_gram_reduce(q.Gp, nblk, b, G);
+  if (PASS == 0) {
+    if (tid == 0) {
+      double tr = 0.0;
+      for (int i = 0; i < b; ++i) tr += G[i * kCqLd + i];
+      *q.pnorm = sqrt(tr);
+    }
+  } else {
+    for (int e = tid; e < kCqG; e += 256) {
+      const int i = e >> 5, c = e & 31;
+      if (c <= i && i < b && !(fabs(G[i * kCqLd + c] - (i == c ? 1.0 : 0.0)) <= 0.1)) okf = 0;
+    }
+    __syncthreads();
+  }
+  if (tid < 64) {
+    const bool ok = wv_cholesky(G, Lo, rd);
+    if (tid == 0 && !ok) okf = 0;
+  }
+  __syncthreads();
+  if (!okf) {
+    if (tid == 0) { *flag = 1; *q.cnt = 0; }
+    return;
+  }
+  if (tid < 64) wv_lower_inverse(Lo, rd, Li);
+  __syncthreads();
+  if (PASS == 0) {
+    cq_store(Li, q.Li);
+    cq_store(Lo, q.L1s);
+    if (tid == 0) { *flag = 0; *q.cnt = 0; }
+    return;
+  }
+  // ---- after the second factor: R, then the reconstruction from Q1
+  __shared__ double Ma[kTsNB * kCqLd];
+  __shared__ double Mb[kTsNB * kCqLd];
+  __shared__ double B[kTsNB * kCqLd];
+  __shared__ double U[kTsNB * kCqLd];
+  __shared__ double Lf[kTsNB * kCqLd];
+  __shared__ double sg[kTsNB];
+  __shared__ double rdU[kTsNB];
+  cq_load(q.L1s, Ma);
+  // top b rows of Q1 (written by the first workgroup above) -> X rows 0..31
+  for (int e = tid; e < kCqG; e += 256) {
+    const int i = e >> 5, c = e & 31;
+    X[i * kCqLd + c] = i < b && c < b ? q.Qb[(size_t)i * kTsNB + c] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < kCqG; e += 256) {  // R[i][c] = (L1 L2)[c][i] -> Mb (upper)
+    const int i = e >> 5, c = e & 31;
+    double acc = 0.0;
+    if (i < b && c < b && i <= c)
+      for (int k = i; k <= c; ++k) acc += Ma[c * kCqLd + k] * Lo[k * kCqLd + i];
+    Mb[i * kCqLd + c] = acc;
+  }
+  cq_rows_transform(X, Li, B, kTsNB, b);  // B = the top b x b of Q
+  __syncthreads();
+  cq_store(Mb, q.Rs);
+  cq_store(Li, q.Li);
+  if (tid < 64) wv_lu(B, Lf, U, sg, rdU);
+  __syncthreads();
+  if (tid < 64) wv_lower_inverse(Lf, nullptr, Ma);  // Ma = L1f^-1
+  else if (tid < 128) wv_upper_inverse(U, rdU, G);  // G = U^-1
+  __syncthreads();
+  cq_store(Lf, q.L1f);
+  cq_store(G, q.Ui);
+  for (int e = tid; e < kCqG; e += 256) {
+    const int i = e >> 5, j = e & 31;
+    double t = 0.0;
+    if (i < b && j < b && i <= j)
+      for (int k = i; k <= j; ++k) t += U[i * kCqLd + k] * sg[k] * Ma[j * kCqLd + k];
+    q.T[e] = -t;
+    q.Rbuf[e] = i < b && j < b && j >= i ? sg[i] * Mb[i * kCqLd + j] : 0.0;
+  }
+  if (tid == 0) *q.cnt = 0;
+}
+
+// Rows of Q = Q1 R2^-1, the check of P - Q R, and the rows of V, Y1 = V T^T
+// and Y2 = V T (V's top b rows are L1 of the LU). A workgroup whose rows
+// fail the check sets the flag.
+__global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__ A, int n, int jb,
+                                                     int mp, int b, CqBuf q,
+                                                     double* __restrict__ Vall,
+                                                     double* __restrict__ Y1,
+                                                     double* __restrict__ Y2,
+                                                     int* __restrict__ flag) {
+  __shared__ double X[kCqRB * kCqLd];
+  __shared__ double Qr[kCqRB * kCqLd];
+  __shared__ double Li[kTsNB * kCqLd];
+  __shared__ double Ui[kTsNB * kCqLd];
+  __shared__ double T[kTsNB * kCqLd];
+  __shared__ double Lf[kTsNB * kCqLd];
+  __shared__ double Rs[kTsNB * kCqLd];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (*flag) return;
+  const int r0 = blockIdx.x * kCqRB;
+  if (tid == 0) bad = 0;
+  cq_load(q.Li, Li);
+  cq_load(q.Ui, Ui);
+  cq_load(q.T, T);
+  cq_load(q.L1f, Lf);
+  cq_load(q.Rs, Rs);
+  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+    const int i = e >> 5, c = e & 31;
+    X[i * kCqLd + c] = r0 + i < mp && c < b ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
+  }
+  __syncthreads();
+  cq_rows_transform(X, Li, Qr, kCqRB, b);
+  __syncthreads();
+  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+    const int i = e >> 5, c = e & 31;
+    X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+  }
+  __syncthreads();
+  const int row = tid >> 2, part = tid & 3;
+  const int pr = r0 + row;
+  {
+    const double thr = 2e-14 * *q.pnorm;
+    bool fail = false;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int c = part + 4 * t;
+      if (c >= b) continue;
+      double acc = X[row * kCqLd + c];
+      for (int k = 0; k <= c; ++k) acc -= Qr[row * kCqLd + k] * Rs[k * kCqLd + c];
+      if (!(fabs(acc) <= thr)) fail = true;
+    }
+    if (fail) bad = 1;
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) *flag = 1;
+    return;
+  }
+  // V rows -> X
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int c = part + 4 * t;
+    double v = 0.0;
+    if (c < b) {
+      if (pr < b) {
+        v = c < pr ? Lf[pr * kCqLd + c] : (c == pr ? 1.0 : 0.0);
+      } else {
+        for (int k = 0; k <= c; ++k) v += Qr[row * kCqLd + k] * Ui[k * kCqLd + c];
+      }
+    }
+    X[row * kCqLd + c] = v;
+  }
+  __syncthreads();
+  if (pr < mp) {
+    const int64_t off = (int64_t)(jb + pr) * n + jb;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int c = part + 4 * t;
+      if (c >= b) continue;
+      double y1 = 0.0, y2 = 0.0;
+      for (int k = 0; k < b; ++k) {
+        const double v = X[row * kCqLd + k];
+        y1 += v * T[c * kCqLd + k];
+        y2 += v * T[k * kCqLd + c];
+      }
+      Vall[off + c] = X[row * kCqLd + c];
+      Y1[off + c] = y1;
+      Y2[off + c] = y2;
+    }
+  }
+}
+
 static bool ts_fits(int m, int n) {
   TsPlan p;
   return m >= n && ts_plan(m, &p);  // the first panel has the most rows
@@ -564,6 +1044,11 @@ static size_t ts_ws_bytes(int m, int n) {
   b += al((size_t)kTsNB * n * 8);                        // W
   b += al((size_t)32 * kTsNB * n * 8);                   // split-K partials
   b += al((size_t)n * 8);                                // R diagonal signs
+  b += al((size_t)m * kTsNB * 8);                        // Cholesky-QR panel rows
+  b += al((size_t)((m + kCqRB - 1) / kCqRB) * kCqG * 8);  // Gram partials
+  b += 7 * al((size_t)kCqG * 8);                         // Li, L1, R, S R, L1f, U^-1, T
+  b += al(8) + al(sizeof(int));                          // ||P||_F, arrival ticket
+  b += al((size_t)((n + kTsNB - 1) / kTsNB) * sizeof(int));    // per-panel fallback flags
   return b + 256;
 }
 
@@ -614,7 +1099,17 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   double* Ust = (double*)p; p += al((size_t)kTsNB * kTsNB * 8);
   double* Wb = (double*)p; p += al((size_t)kTsNB * n * 8);
   double* part = (double*)p; p += al((size_t)32 * kTsNB * n * 8);
-  double* sg = (double*)p;
+  double* sg = (double*)p; p += al((size_t)n * 8);
+  CqBuf q;
+  q.Qb = (double*)p; p += al((size_t)m * kTsNB * 8);
+  q.Gp = (double*)p; p += al((size_t)((m + kCqRB - 1) / kCqRB) * kCqG * 8);
+  double** smalls[7] = {&q.Li, &q.L1s, &q.Rs, &q.Rbuf, &q.L1f, &q.Ui, &q.T};
+  for (double** sm : smalls) { *sm = (double*)p; p += al((size_t)kCqG * 8); }
+  q.pnorm = (double*)p; p += al(8);
+  q.cnt = (int*)p; p += al(sizeof(int));
+  int* flags = (int*)p;
+  hipLaunchKernelGGL(cq_init_kernel, dim3(1), dim3(1), 0, s, q.cnt);
+  TNX_LAUNCH_CHECK();
   const size_t part_bytes = al((size_t)32 * kTsNB * n * 8);
   const int np = (n + kTsNB - 1) / kTsNB;
   for (int pi = 0; pi < np; ++pi) {
@@ -626,14 +1121,26 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
       set_error("qr: panel of %d rows has no TSQR plan", mp);
       return TNX_ERR_UNSUPPORTED;
     }
+    // Cholesky-QR passes; the TSQR kernels run only for a panel they flag
+    const int nblk = (mp + kCqRB - 1) / kCqRB;
+    int* fl = flags + pi;
+    hipLaunchKernelGGL(cq_pass_kernel<0>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp,
+                       b, q, nblk, fl);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cq_pass_kernel<1>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp,
+                       b, q, nblk, fl);
+    TNX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cq_rows_kernel, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp, b,
+                       q, Vall, Y1, Y2, fl);
+    TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b, pl.h,
-                       pl.nleaf, Rst, Qst);
+                       pl.nleaf, Rst, Qst, fl);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tsqr_root_kernel, dim3(1), dim3(512), 0, s, Rst, pl.nleaf, b, Qst, A, n, jb,
-                       Vall, Tst, Qtop, Ust);
+                       Vall, Tst, Qtop, Ust, fl, (const double*)q.Rbuf);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tsqr_apply_kernel, dim3(pl.nleaf), dim3(256), 0, s, Qst, Qtop, Ust, Tst, mp,
-                       b, pl.h, pl.nleaf, n, jb, Vall, Y1, Y2);
+                       b, pl.h, pl.nleaf, n, jb, Vall, Y1, Y2, fl);
     TNX_LAUNCH_CHECK();
     const int nt = n - jb - b;
     if (nt <= 0) continue;

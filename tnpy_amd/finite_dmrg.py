@@ -434,11 +434,17 @@ class ShiftInvertDMRG(FiniteDMRG):
         offset: eps.
         block_size, mps, exact_solver_dim: as for ``FiniteDMRG`` (one-site
             updates only).
+        dense_dim: local dimensions up to this are also solved densely.
 
     Per site the generalized eigenproblem is solved on the GPU: dense
-    (``tnx_sygv_lowest``) below ``exact_solver_dim``, generalized Davidson
-    (``tnx_eigsh_gen_heff1``) above, with a second environment of the
-    squared MPO (D^2 bond) kept next to the first.
+    (``tnx_sygv_lowest``) below ``exact_solver_dim`` and up to
+    ``dense_dim``, generalized Davidson (``tnx_eigsh_gen_heff1``) above,
+    with a second environment of the squared MPO (D^2 bond) kept next to
+    the first. The pencil's eigenvalues 1 / (E - eps) lie on both sides of
+    the target with the extremes of largest magnitude next to it, so an
+    unpreconditioned Krylov iteration converges slowly on it (thousands of
+    matvecs at local dimension ~10^3); the dense solve is exact and, at
+    these sizes, cheaper.
     """
 
     def __init__(
@@ -455,6 +461,7 @@ class ShiftInvertDMRG(FiniteDMRG):
         max_basis: int = 64,
         keep: int = 8,
         compute_variance: bool = True,
+        dense_dim: int = 2048,
         device: Optional[torch.device] = None,
     ):
         if block_size != 1:
@@ -466,6 +473,7 @@ class ShiftInvertDMRG(FiniteDMRG):
         )
         self._env2 = Environment(mpo=self._env.mpo.square(), mps=self.mps)
         self._offset = offset
+        self._dense_dim = int(dense_dim)
         self._restored_mps: Optional[MatrixProductState] = None
 
     @property
@@ -491,7 +499,7 @@ class ShiftInvertDMRG(FiniteDMRG):
         _, D2l, D2r, _, _ = env2.site_dims(site)
         vec = torch.empty(N, dtype=torch.float64, device=self.device)
         hvec = torch.empty(N, dtype=torch.float64, device=self.device)
-        if N < self._exact_solver_dim:
+        if N < self._exact_solver_dim or N <= self._dense_dim:
             # scipy.linalg.eigh(a, b) branch of linalg.eigh (tnpy/linalg.py:42-61)
             Ma = env.full_matrix_device(site)
             Mb = env2.full_matrix_device(site)
