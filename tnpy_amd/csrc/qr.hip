@@ -616,7 +616,7 @@ __device__ __forceinline__ double qr_lane_val(double v, int i) {
 
 // True in the workgroup that finishes last; its reads then see every other
 // workgroup's writes made before their arrival.
-__device__ bool cq_arrive_last(int* cnt) {
+__device__ __forceinline__ bool cq_arrive_last(int* cnt) {
   __shared__ int last;
   __threadfence();
   __syncthreads();
@@ -628,7 +628,7 @@ __device__ bool cq_arrive_last(int* cnt) {
 
 // G = sum of nblk partial Grams, lower triangle, fixed order per entry, 16
 // loads in flight per thread; entries outside b x b are the identity's.
-__device__ void cq_gram_reduce(const double* __restrict__ Gp, int nblk, int b, double* G) {
+__device__ __forceinline__ void cq_gram_reduce(const double* __restrict__ Gp, int nblk, int b, double* G) {
   int e[4];
   bool on[4];
   double s[4];
@@ -659,12 +659,39 @@ __device__ void cq_gram_reduce(const double* __restrict__ Gp, int nblk, int b, d
   __syncthreads();
 }
 
+// 1/sqrt(x) and 1/a: the hardware estimates refined by two Newton steps
+// (shorter dependent chains than the correctly rounded sequences; the
+// results are within an ulp or two, which the factorisations below absorb).
+__device__ __forceinline__ double qr_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+__device__ __forceinline__ double qr_rcp(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return fma(r, fma(-a, r, 1.0), r);
+}
+
+// Orders one wave's LDS write before its later reads of other lanes' values
+// (LDS serves a wave's instructions in order; this keeps the compiler from
+// moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Cholesky G = L L^T (lower triangle of G in LDS) by the calling wave: row i
-// in lane i's registers (lanes >= kTsNB mirror row kTsNB - 1), column j's
-// entries broadcast with v_readlane. L (zeros above the diagonal) to Lo,
+// in lane i's registers (lanes >= kTsNB mirror row kTsNB - 1). The pivot is
+// read with v_readlane; column j of L goes to the LDS buffer cb and every
+// lane reads the entries it needs. L (zeros above the diagonal) to Lo,
 // 1/L_jj to rd. Returns false when a pivot is not positive and finite (the
 // same in every lane).
-__device__ bool wv_cholesky(const double* G, double* Lo, double* rd) {
+__device__ __forceinline__ bool wv_cholesky(const double* G, double* Lo, double* rd, double* cb) {
   const int lane = threadIdx.x & 63;
   const int i = lane < kTsNB ? lane : kTsNB - 1;
   double g[kTsNB];
@@ -675,12 +702,14 @@ __device__ bool wv_cholesky(const double* G, double* Lo, double* rd) {
   for (int j = 0; j < kTsNB; ++j) {
     const double dj = qr_lane_val(g[j], j);
     ok = ok && dj > 0.0 && dj <= 1.7976931348623157e308;
-    const double sq = sqrt(dj);
-    const double rs = 1.0 / sq;
-    const double l = i > j ? g[j] * rs : (i == j ? sq : 0.0);
+    const double rs = qr_rsqrt(dj);
+    const double l = i > j ? g[j] * rs : (i == j ? dj * rs : 0.0);
     g[j] = l;
+    if (lane < kTsNB) cb[lane] = l;
+    wave_lds_sync();
 #pragma unroll
-    for (int c = j + 1; c < kTsNB; ++c) g[c] = fma(-l, qr_lane_val(l, c), g[c]);
+    for (int c = j + 1; c < kTsNB; ++c) g[c] = fma(-l, cb[c], g[c]);
+    wave_lds_sync();
     if (lane == 0) rd[j] = rs;
   }
   if (lane < kTsNB) {
@@ -691,9 +720,10 @@ __device__ bool wv_cholesky(const double* G, double* Lo, double* rd) {
 }
 
 // LU of B - S without pivoting (B in LDS), S_jj = -sign of the pivot so
-// |U_jj| >= 1, by the calling wave with row i in lane i: the unit lower
-// factor to Lf, U to U (zeros elsewhere), S to sg, 1/U_jj to rdU.
-__device__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double* rdU) {
+// |U_jj| >= 1, by the calling wave with row i in lane i; row j of U goes
+// to the LDS buffer cb for the other lanes. The unit lower factor to Lf, U
+// to U (zeros elsewhere), S to sg, 1/U_jj to rdU.
+__device__ __forceinline__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double* rdU, double* cb) {
   const int lane = threadIdx.x & 63;
   const int i = lane < kTsNB ? lane : kTsNB - 1;
   double g[kTsNB];
@@ -704,15 +734,20 @@ __device__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double
     const double a = qr_lane_val(g[j], j);
     const double sgn = a >= 0.0 ? -1.0 : 1.0;
     const double piv = a - sgn;
-    const double rp = 1.0 / piv;
-    const double l = g[j] * rp;
+    const double rp = qr_rcp(piv);
+    if (lane == j) {
 #pragma unroll
-    for (int c = j + 1; c < kTsNB; ++c) {
-      const double u = qr_lane_val(g[c], j);  // U[j][c]: row j is final
-      if (i > j) g[c] = fma(-l, u, g[c]);
+      for (int c = j + 1; c < kTsNB; ++c) cb[c] = g[c];
     }
-    if (i > j) g[j] = l;
+    wave_lds_sync();
+    const double l = g[j] * rp;
+    if (i > j) {
+#pragma unroll
+      for (int c = j + 1; c < kTsNB; ++c) g[c] = fma(-l, cb[c], g[c]);
+      g[j] = l;
+    }
     if (i == j) g[j] = piv;
+    wave_lds_sync();
     if (lane == 0) { sg[j] = sgn; rdU[j] = rp; }
   }
   if (lane < kTsNB) {
@@ -727,7 +762,7 @@ __device__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double
 // Li = L^-1 for lower triangular L (LDS; 1/L_ii in rd, or unit diagonal
 // when rd is null): column c in lane c of the calling wave, four
 // accumulation chains per entry. Zeros above the diagonal.
-__device__ void wv_lower_inverse(const double* L, const double* rd, double* Li) {
+__device__ __forceinline__ void wv_lower_inverse(const double* L, const double* rd, double* Li) {
   const int c = threadIdx.x & 63;
   double x[kTsNB];
 #pragma unroll
@@ -753,7 +788,7 @@ __device__ void wv_lower_inverse(const double* L, const double* rd, double* Li) 
 
 // Ui = U^-1 for upper triangular U (LDS; 1/U_ii in rd): column c in lane c
 // of the calling wave. Zeros below the diagonal.
-__device__ void wv_upper_inverse(const double* U, const double* rd, double* Ui) {
+__device__ __forceinline__ void wv_upper_inverse(const double* U, const double* rd, double* Ui) {
   const int c = threadIdx.x & 63;
   double x[kTsNB];
 #pragma unroll
@@ -778,7 +813,7 @@ __device__ void wv_upper_inverse(const double* U, const double* rd, double* Ui) 
 
 // Y[row] = X[row] Li^T for nrows rows (Y = X R^-1 with R = L^T), four
 // threads per row, eight columns each; columns >= b are written as zero.
-__device__ void cq_rows_transform(const double* X, const double* Li, double* Y, int nrows, int b) {
+__device__ __forceinline__ void cq_rows_transform(const double* X, const double* Li, double* Y, int nrows, int b) {
   const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
   if (row >= nrows) return;
   double y[8];
@@ -798,7 +833,7 @@ __device__ void cq_rows_transform(const double* X, const double* Li, double* Y, 
 
 // Partial Gram Y^T Y (lower triangle) of the kCqRB rows in LDS (rows past
 // the panel are zero).
-__device__ void cq_gram_partial(const double* Y, double* __restrict__ Gout) {
+__device__ __forceinline__ void cq_gram_partial(const double* Y, double* __restrict__ Gout) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = threadIdx.x + 256 * q;
@@ -814,11 +849,11 @@ __device__ void cq_gram_partial(const double* Y, double* __restrict__ Gout) {
   }
 }
 
-__device__ void cq_store(const double* S, double* __restrict__ g) {
+__device__ __forceinline__ void cq_store(const double* S, double* __restrict__ g) {
   for (int e = threadIdx.x; e < kCqG; e += 256) g[e] = S[(e >> 5) * kCqLd + (e & 31)];
 }
 
-__device__ void cq_load(const double* __restrict__ g, double* S) {
+__device__ __forceinline__ void cq_load(const double* __restrict__ g, double* S) {
   for (int e = threadIdx.x; e < kCqG; e += 256) S[(e >> 5) * kCqLd + (e & 31)] = g[e];
 }
 
@@ -840,6 +875,7 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   __shared__ double Lo[kTsNB * kCqLd];
   __shared__ double Li[kTsNB * kCqLd];
   __shared__ double rd[kTsNB];
+  __shared__ double cb[kTsNB];
   __shared__ int okf;
   const int tid = threadIdx.x;
   if (PASS > 0 && *flag) return;
@@ -878,7 +914,7 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
     __syncthreads();
   }
   if (tid < 64) {
-    const bool ok = wv_cholesky(G, Lo, rd);
+    const bool ok = wv_cholesky(G, Lo, rd, cb);
     if (tid == 0 && !ok) okf = 0;
   }
   __syncthreads();
@@ -920,7 +956,7 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   __syncthreads();
   cq_store(Mb, q.Rs);
   cq_store(Li, q.Li);
-  if (tid < 64) wv_lu(B, Lf, U, sg, rdU);
+  if (tid < 64) wv_lu(B, Lf, U, sg, rdU, cb);
   __syncthreads();
   if (tid < 64) wv_lower_inverse(Lf, nullptr, Ma);  // Ma = L1f^-1
   else if (tid < 128) wv_upper_inverse(U, rdU, G);  // G = U^-1
