@@ -106,6 +106,9 @@ def parse():
     p.add_argument("--kernels-only", action="store_true",
                    help="only the HIP-event kernel rates at chi, plus a plain 256 MiB device copy "
                         "(the reference kernel for scaling HBM counters)")
+    p.add_argument("--truncation", type=int, default=0, metavar="R",
+                   help="only the two-site truncation: R timed SVDs of a (2 chi) x (2 chi) block "
+                        "with a known spectrum cut to chi")
     return p.parse_args()
 
 
@@ -366,6 +369,41 @@ def cpu_baseline(chi, D, d, n, mv_per_site, bond, reps):
     return {"sweep_s": total, "t_matvec_s": t_mv, "t_svd_s": t_svd, "t_env_s": t_env}
 
 
+def truncation_leg(chi, reps, dev):
+    """The two-site truncation alone: a (2 chi) x (2 chi) block with singular
+    values logspace(0, -12) between seeded random orthogonal factors, cut to
+    chi by the library SVD, timed with HIP events on the library's stream;
+    the kept values against the known spectrum."""
+    import numpy as np
+    import torch
+
+    from tnpy_amd import _lib
+
+    n, k = 2 * chi, chi
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    Ua, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    Vb, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    s_true = np.logspace(0, -12, n)
+    X = (Ua * torch.as_tensor(s_true, device=dev)) @ Vb.T
+    ws = _lib.Workspace(dev)
+    _lib.svd_trunc(X, k, ws)  # workspace and tile set-up outside the timed calls
+    torch.cuda.synchronize()
+    times, err = [], 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _, _, sv = _lib.svd_trunc(X, k, ws)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+        err = max(err, float(np.abs(sv - s_true[:k]).max()))
+    log(f"truncation {n}x{n} -> {k}: best {min(times):.1f} ms, mean {np.mean(times):.1f} ms, "
+        f"max |s - s_true| {err:.2e}")
+    return {"block": [n, n], "keep": k, "reps": reps, "best_ms": min(times), "mean_ms": float(np.mean(times)),
+            "max_abs_sv_err": err}
+
+
 def two_site_leg(args, dev):
     """Two-site DMRG sample: random-field Heisenberg chain (Delta=1, h_i ~
     U(-0.5, 0.5) from the seed), seeded random MPS at chi, orthogonality
@@ -572,6 +610,9 @@ def main():
         return
     if args.two_site > 0:
         print(json.dumps({"two_site": two_site_leg(args, dev)}), flush=True)
+        return
+    if args.truncation > 0:
+        print(json.dumps({"truncation": truncation_leg(args.chi, args.truncation, dev)}), flush=True)
         return
     if args.kernels_only:
         rates = kernel_rates(args.chi, 5, 2, ["fp64", args.mixed])

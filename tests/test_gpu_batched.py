@@ -287,9 +287,11 @@ def _chains(ns, seed):
     return [RandomCouplingHeisenberg(n=n, seed=int(rng.integers(1 << 30))) for n in ns]
 
 
-def test_batched_dmrg_matches_single(gpu):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_batched_dmrg_matches_single(gpu, concurrent):
     """Per chain, sweep by sweep, the batched driver equals FiniteDMRG from
-    the same initial state (mixed lengths, dense and Krylov sites)."""
+    the same initial state (mixed lengths, dense and Krylov sites), with the
+    length groups swept one after another and concurrently."""
     from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
     from tnpy_amd.batched_dmrg import BatchedFiniteDMRG
 
@@ -298,7 +300,7 @@ def test_batched_dmrg_matches_single(gpu):
     models = _chains(ns, 7)
     starts = [MatrixProductState.random(n, chi, 2, seed=100 + i, device=gpu) for i, n in enumerate(ns)]
     bd = BatchedFiniteDMRG([m.mpo for m in models], bond_dim=chi,
-                           mps=[s.copy() for s in starts], device=gpu)
+                           mps=[s.copy() for s in starts], device=gpu, concurrent=concurrent)
     assert bd.groups == {10: [0, 2], 12: [4], 14: [1, 3]}
     dirs = [Direction.RIGHTWARD, Direction.LEFTWARD, Direction.RIGHTWARD]
     eb = np.array([bd.sweep(dr, tol=1e-12) for dr in dirs]).T

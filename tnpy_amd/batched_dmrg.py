@@ -250,7 +250,10 @@ class BatchedFiniteDMRG:
         exact_solver_dim, max_basis, keep: as for ``FiniteDMRG``.
         max_matvecs: Lanczos matvec limit per site.
         concurrent: sweep the length groups concurrently (one host thread
-            and stream each); False runs them one after another.
+            and stream each). Off by default: the groups run one after
+            another (the same kernels), because the concurrent form has
+            ended in a host fault inside the launch path under a kernel
+            tracer, a cause not yet found.
     """
 
     def __init__(
@@ -265,7 +268,7 @@ class BatchedFiniteDMRG:
         max_basis: int = 64,
         keep: int = 20,
         max_matvecs: int = 5000,
-        concurrent: bool = True,
+        concurrent: bool = False,
         device: Optional[torch.device] = None,
     ):
         if chi is not None and bond_dim is not None and chi != bond_dim:

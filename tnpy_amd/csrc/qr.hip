@@ -659,23 +659,6 @@ __device__ __forceinline__ void cq_gram_reduce(const double* __restrict__ Gp, in
   __syncthreads();
 }
 
-// 1/sqrt(x) and 1/a: the hardware estimates refined by two Newton steps
-// (shorter dependent chains than the correctly rounded sequences; the
-// results are within an ulp or two, which the factorisations below absorb).
-__device__ __forceinline__ double qr_rsqrt(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double e = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, e, y);
-  e = fma(-x * y, y, 1.0);
-  return fma(0.5 * y, e, y);
-}
-
-__device__ __forceinline__ double qr_rcp(double a) {
-  double r = __builtin_amdgcn_rcp(a);
-  r = fma(r, fma(-a, r, 1.0), r);
-  return fma(r, fma(-a, r, 1.0), r);
-}
-
 // Orders one wave's LDS write before its later reads of other lanes' values
 // (LDS serves a wave's instructions in order; this keeps the compiler from
 // moving them).
@@ -702,7 +685,7 @@ __device__ __forceinline__ bool wv_cholesky(const double* G, double* Lo, double*
   for (int j = 0; j < kTsNB; ++j) {
     const double dj = qr_lane_val(g[j], j);
     ok = ok && dj > 0.0 && dj <= 1.7976931348623157e308;
-    const double rs = qr_rsqrt(dj);
+    const double rs = rsqrt_nr(dj);
     const double l = i > j ? g[j] * rs : (i == j ? dj * rs : 0.0);
     g[j] = l;
     if (lane < kTsNB) cb[lane] = l;
@@ -734,7 +717,7 @@ __device__ __forceinline__ void wv_lu(const double* B, double* Lf, double* U, do
     const double a = qr_lane_val(g[j], j);
     const double sgn = a >= 0.0 ? -1.0 : 1.0;
     const double piv = a - sgn;
-    const double rp = qr_rcp(piv);
+    const double rp = rcp_nr(piv);
     if (lane == j) {
 #pragma unroll
       for (int c = j + 1; c < kTsNB; ++c) cb[c] = g[c];
@@ -975,16 +958,19 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
 }
 
 // Rows of Q = Q1 R2^-1, the check of P - Q R, and the rows of V, Y1 = V T^T
-// and Y2 = V T (V's top b rows are L1 of the LU). A workgroup whose rows
-// fail the check sets the flag.
+// and Y2 = V T (V's top b rows are L1 of the LU), kCqRows rows per
+// workgroup with 16 threads per row (columns part and part + 16). A
+// workgroup whose rows fail the check sets the flag.
+constexpr int kCqRows = 16;
+
 __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__ A, int n, int jb,
                                                      int mp, int b, CqBuf q,
                                                      double* __restrict__ Vall,
                                                      double* __restrict__ Y1,
                                                      double* __restrict__ Y2,
                                                      int* __restrict__ flag) {
-  __shared__ double X[kCqRB * kCqLd];
-  __shared__ double Qr[kCqRB * kCqLd];
+  __shared__ double X[kCqRows * kCqLd];
+  __shared__ double Qr[kCqRows * kCqLd];
   __shared__ double Li[kTsNB * kCqLd];
   __shared__ double Ui[kTsNB * kCqLd];
   __shared__ double T[kTsNB * kCqLd];
@@ -993,36 +979,46 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (*flag) return;
-  const int r0 = blockIdx.x * kCqRB;
+  const int r0 = blockIdx.x * kCqRows;
   if (tid == 0) bad = 0;
   cq_load(q.Li, Li);
   cq_load(q.Ui, Ui);
   cq_load(q.T, T);
   cq_load(q.L1f, Lf);
   cq_load(q.Rs, Rs);
-  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+  for (int e = tid; e < kCqRows * kTsNB; e += 256) {
     const int i = e >> 5, c = e & 31;
     X[i * kCqLd + c] = r0 + i < mp && c < b ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
   }
   __syncthreads();
-  cq_rows_transform(X, Li, Qr, kCqRB, b);
+  const int row = tid >> 4, part = tid & 15;
+  const int pr = r0 + row;
+  const int c0 = part, c1 = part + 16;
+  {  // Q = X Li^T
+    double y0 = 0.0, y1 = 0.0;
+    for (int k = 0; k < b; ++k) {
+      const double xv = X[row * kCqLd + k];
+      y0 = fma(xv, Li[c0 * kCqLd + k], y0);
+      y1 = fma(xv, Li[c1 * kCqLd + k], y1);
+    }
+    Qr[row * kCqLd + c0] = c0 < b ? y0 : 0.0;
+    Qr[row * kCqLd + c1] = c1 < b ? y1 : 0.0;
+  }
   __syncthreads();
-  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
+  for (int e = tid; e < kCqRows * kTsNB; e += 256) {
     const int i = e >> 5, c = e & 31;
     X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
   }
   __syncthreads();
-  const int row = tid >> 2, part = tid & 3;
-  const int pr = r0 + row;
-  {
+  {  // |P - Q R| against 2e-14 ||P||_F
     const double thr = 2e-14 * *q.pnorm;
     bool fail = false;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int c = part + 4 * t;
+    for (int t = 0; t < 2; ++t) {
+      const int c = t ? c1 : c0;
       if (c >= b) continue;
       double acc = X[row * kCqLd + c];
-      for (int k = 0; k <= c; ++k) acc -= Qr[row * kCqLd + k] * Rs[k * kCqLd + c];
+      for (int k = 0; k <= c; ++k) acc = fma(-Qr[row * kCqLd + k], Rs[k * kCqLd + c], acc);
       if (!(fabs(acc) <= thr)) fail = true;
     }
     if (fail) bad = 1;
@@ -1034,14 +1030,14 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
   }
   // V rows -> X
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int c = part + 4 * t;
+  for (int t = 0; t < 2; ++t) {
+    const int c = t ? c1 : c0;
     double v = 0.0;
     if (c < b) {
       if (pr < b) {
         v = c < pr ? Lf[pr * kCqLd + c] : (c == pr ? 1.0 : 0.0);
       } else {
-        for (int k = 0; k <= c; ++k) v += Qr[row * kCqLd + k] * Ui[k * kCqLd + c];
+        for (int k = 0; k <= c; ++k) v = fma(Qr[row * kCqLd + k], Ui[k * kCqLd + c], v);
       }
     }
     X[row * kCqLd + c] = v;
@@ -1050,14 +1046,14 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
   if (pr < mp) {
     const int64_t off = (int64_t)(jb + pr) * n + jb;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int c = part + 4 * t;
+    for (int t = 0; t < 2; ++t) {
+      const int c = t ? c1 : c0;
       if (c >= b) continue;
       double y1 = 0.0, y2 = 0.0;
       for (int k = 0; k < b; ++k) {
         const double v = X[row * kCqLd + k];
-        y1 += v * T[c * kCqLd + k];
-        y2 += v * T[k * kCqLd + c];
+        y1 = fma(v, T[c * kCqLd + k], y1);
+        y2 = fma(v, T[k * kCqLd + c], y2);
       }
       Vall[off + c] = X[row * kCqLd + c];
       Y1[off + c] = y1;
@@ -1166,8 +1162,8 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
     hipLaunchKernelGGL(cq_pass_kernel<1>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp,
                        b, q, nblk, fl);
     TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cq_rows_kernel, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp, b,
-                       q, Vall, Y1, Y2, fl);
+    hipLaunchKernelGGL(cq_rows_kernel, dim3((mp + kCqRows - 1) / kCqRows), dim3(256), 0, s,
+                       (const double*)A, n, jb, mp, b, q, Vall, Y1, Y2, fl);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b, pl.h,
                        pl.nleaf, Rst, Qst, fl);

@@ -52,9 +52,13 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // One sweep of two-sided cyclic Jacobi (round-robin, kJp - 1 rounds of kJw
 // disjoint rotations) on each pair's Gram matrix G (kJp x kJp, row-major);
-// Z accumulates the rotations (G_final = Z^T G Z). Also the pair's
-// off-diagonal measure, max-reduced into *off_bits (a non-negative double
-// read as an unsigned integer, so integer max = double max).
+// Z accumulates the rotations (G_final = Z^T G Z). A round's kJw rotations
+// partition the kJp indices, so G <- J^T G J is one pass over 2 x 2 blocks
+// (rows {p_k, q_k}, columns {p_l, q_l} of rotations k and l) with no two
+// threads touching one entry, and Z <- Z J one pass over row pairs: two
+// barriers per round. Also the pair's off-diagonal measure, max-reduced
+// into *off_bits (a non-negative double read as an unsigned integer, so
+// integer max = double max).
 __global__ __launch_bounds__(256) void jacobi_pair_kernel(const double* __restrict__ Gall,
                                                          double* __restrict__ Zall,
                                                          const double* __restrict__ floor2,
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(const double* __restri
   const int tid = threadIdx.x;
   const double* Gp = Gall + (size_t)blockIdx.x * kJp * kJp;
   for (int e = tid; e < kJp * kJp; e += 256) {
-    const int i = e / kJp, j = e - i * kJp;
+    const int i = e >> 6, j = e & (kJp - 1);
     // symmetrise (the GEMM's two halves round independently)
     G[i * kJld + j] = 0.5 * (Gp[e] + Gp[j * kJp + i]);
     Z[i * kJld + j] = i == j ? 1.0 : 0.0;
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(const double* __restri
     const double fl = floor2[0];
     double mx = 0.0;
     for (int e = tid; e < kJp * kJp; e += 256) {
-      const int i = e / kJp, j = e - i * kJp;
+      const int i = e >> 6, j = e & (kJp - 1);
       if (i != j) {
         const double den = fmax(sqrt(fmax(G[i * kJld + i], 0.0) * fmax(G[j * kJld + j], 0.0)), fl);
         mx = fmax(mx, fabs(G[i * kJld + j]) / den);
@@ -107,49 +111,56 @@ __global__ __launch_bounds__(256) void jacobi_pair_kernel(const double* __restri
       const double a = G[p * kJld + p], d = G[q * kJld + q], b = G[p * kJld + q];
       double c = 1.0, s = 0.0;
       if (b != 0.0) {
+        // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (d - a) / (2 b);
+        // the refined reciprocals only see arguments >= 1
         const double zeta = (d - a) / (2.0 * b);
         double t;
-        if (fabs(zeta) > 1e100) t = 0.5 / zeta;
-        else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        c = 1.0 / sqrt(1.0 + t * t);
+        if (fabs(zeta) > 1e100) {
+          t = 0.5 / zeta;
+        } else {
+          const double w = fma(zeta, zeta, 1.0);
+          t = (zeta >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + w * rsqrt_nr(w));
+        }
+        c = rsqrt_nr(fma(t, t, 1.0));
         s = c * t;
       }
       cs[i] = c; sn[i] = s; pp[i] = p; qq[i] = q;
     }
     __syncthreads();
-    // rows p, q of G  <- J^T [row p; row q]
-    for (int e = tid; e < kJw * kJp; e += 256) {
-      const int k = e / kJp, j = e - k * kJp;
-      const int p = pp[k], q = qq[k];
-      const double c = cs[k], s = sn[k];
-      const double gp = G[p * kJld + j], gq = G[q * kJld + j];
-      G[p * kJld + j] = c * gp - s * gq;
-      G[q * kJld + j] = s * gp + c * gq;
+    // G <- J^T G J, one 2 x 2 block per (k, l)
+    for (int e = tid; e < kJw * kJw; e += 256) {
+      const int k = e >> 5, l = e & (kJw - 1);
+      const int pk = pp[k], qk = qq[k], pl = pp[l], ql = qq[l];
+      const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
+      const double g00 = G[pk * kJld + pl], g01 = G[pk * kJld + ql];
+      const double g10 = G[qk * kJld + pl], g11 = G[qk * kJld + ql];
+      // rows: [r0; r1] = J_k^T [g0; g1]
+      const double r00 = ck * g00 - sk * g10, r01 = ck * g01 - sk * g11;
+      const double r10 = sk * g00 + ck * g10, r11 = sk * g01 + ck * g11;
+      // columns: [c0 c1] = [r0 r1] J_l
+      G[pk * kJld + pl] = cl * r00 - sl * r01;
+      G[pk * kJld + ql] = sl * r00 + cl * r01;
+      G[qk * kJld + pl] = cl * r10 - sl * r11;
+      G[qk * kJld + ql] = sl * r10 + cl * r11;
     }
-    __syncthreads();
-    // columns p, q of G and Z  <- [col p, col q] J
-    for (int e = tid; e < kJw * kJp; e += 256) {
-      const int k = e / kJp, j = e - k * kJp;
-      const int p = pp[k], q = qq[k];
-      const double c = cs[k], s = sn[k];
-      const double gp = G[j * kJld + p], gq = G[j * kJld + q];
-      G[j * kJld + p] = c * gp - s * gq;
-      G[j * kJld + q] = s * gp + c * gq;
-      const double zp = Z[j * kJld + p], zq = Z[j * kJld + q];
-      Z[j * kJld + p] = c * zp - s * zq;
-      Z[j * kJld + q] = s * zp + c * zq;
+    // Z <- Z J
+    for (int e = tid; e < kJp * kJw; e += 256) {
+      const int j = e >> 5, l = e & (kJw - 1);
+      const int pl = pp[l], ql = qq[l];
+      const double cl = cs[l], sl = sn[l];
+      const double zp = Z[j * kJld + pl], zq = Z[j * kJld + ql];
+      Z[j * kJld + pl] = cl * zp - sl * zq;
+      Z[j * kJld + ql] = sl * zp + cl * zq;
     }
     __syncthreads();
   }
   double* Zp = Zall + (size_t)blockIdx.x * kJp * kJp;
   for (int e = tid; e < kJp * kJp; e += 256) {
-    const int i = e / kJp, j = e - i * kJp;
+    const int i = e >> 6, j = e & (kJp - 1);
     Zp[e] = Z[i * kJld + j];
   }
 }
 
-// Block positions for the next round: dst block position of source position
-// p is perm[p]; rows of a block are contiguous (kJw rows of length q).
 __global__ void jacobi_permute_kernel(const double* __restrict__ src, double* __restrict__ dst,
                                       const int* __restrict__ perm, int nblk, int64_t q) {
   const int64_t blk = (int64_t)kJw * q;
@@ -162,12 +173,29 @@ __global__ void jacobi_permute_kernel(const double* __restrict__ src, double* __
   }
 }
 
-// ||X||_F^2 (one block) scaled by the floor factor
-__global__ __launch_bounds__(256) void jacobi_floor_kernel(const double* __restrict__ X, int64_t n,
+// ||X||_F^2 scaled by the floor factor: per-block partial sums of squares,
+// then their sum in a fixed order (deterministic).
+constexpr int kFloorBlocks = 256;
+
+__global__ __launch_bounds__(256) void jacobi_floor_part_kernel(const double* __restrict__ X,
+                                                                int64_t n,
+                                                                double* __restrict__ part) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+    acc = fma(X[e], X[e], acc);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void jacobi_floor_kernel(const double* __restrict__ part, int np,
                                                            double factor, double* __restrict__ out) {
   __shared__ double red[4];
   double acc = 0.0;
-  for (int64_t e = threadIdx.x; e < n; e += 256) acc += X[e] * X[e];
+  for (int e = threadIdx.x; e < np; e += 256) acc += part[e];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -254,7 +282,10 @@ static int jacobi_rows(const double* X0, int p, int q, void* ws, double** Xout, 
   }
   TNX_TRY(fill_f64(X, rows * q, 0.0, s));
   TNX_HIP(hipMemcpyAsync(X, X0, sizeof(double) * (size_t)p * q, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(jacobi_floor_kernel, dim3(1), dim3(256), 0, s, X, (int64_t)p * q, kJacFloor,
+  hipLaunchKernelGGL(jacobi_floor_part_kernel, dim3(kFloorBlocks), dim3(256), 0, s, X,
+                     (int64_t)p * q, part);
+  TNX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(jacobi_floor_kernel, dim3(1), dim3(256), 0, s, part, kFloorBlocks, kJacFloor,
                      floor2);
   TNX_LAUNCH_CHECK();
   const int64_t pq = (int64_t)kJp * q;

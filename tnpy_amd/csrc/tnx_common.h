@@ -107,4 +107,22 @@ int transpose_f64(const double* in, double* out, int rows, int cols,
 int copy_f64(const double* in, double* out, int64_t n, hipStream_t stream);
 int fill_f64(double* out, int64_t n, double v, hipStream_t stream);
 
+// 1/sqrt(x) and 1/a on the device: the hardware estimates (v_rsq_f64,
+// v_rcp_f64) refined by two Newton steps. Shorter dependent chains than the
+// correctly rounded sequences; the results are within an ulp or two, which
+// the factorisations that use them absorb.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+__device__ __forceinline__ double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return fma(r, fma(-a, r, 1.0), r);
+}
+
 }  // namespace tnx
