@@ -190,6 +190,15 @@ def kernel_rates(chi, D, d, precisions):
             "gemm_tflops": FG / tg / 1e9,
             "gemm_flops_per_launch": FG,
         }
+        if p != "fp64":
+            # the form the low-precision Lanczos cycles run: FP32 basis vector
+            # and FP32 environment copy staged directly (FP64 output)
+            A32, B32 = A.float(), B.float()
+            tg32 = event_time(
+                lambda: L.tnx_gemm(chi * d, D * chi, chi, 1, 1.0, A32.data_ptr(), 1, chi, 0, 1, B32.data_ptr(), 1,
+                                   D * chi, 0, 1, 0.0, Cm.data_ptr(), 0, D * chi, 0, pi, s), 20)
+            out[p]["gemm_f32_operands_ms"] = tg32
+            out[p]["gemm_f32_operands_tflops"] = FG / tg32 / 1e9
     # gauge move (QR of the chi*d x chi site matrix, R times the next site)
     site = torch.randn(chi, d, chi, dtype=torch.float64, device=dev, generator=g)
     work = site.clone()
@@ -787,6 +796,16 @@ def main():
             "unit": "TFLOP/s",
             "frac": rates["fp64"]["gemm_tflops"] / PEAK_TFLOPS["fp64"],
             "traffic": measured_traffic("fp64", 2 * chi, 5 * chi, chi),
+        },
+        "roofline_f32_operands": None if "gemm_f32_operands_tflops" not in r else {
+            "bound": "mfma",
+            "kernel": (f"gemm_kernel ({args.mixed}, FP32 operands as in the low-precision Lanczos cycles) "
+                       f"H_eff stage 1, M={2 * chi} N={5 * chi} K={chi}"),
+            "achieved": r["gemm_f32_operands_tflops"],
+            "peak": PEAK_TFLOPS[args.mixed],
+            "unit": "TFLOP/s",
+            "frac": r["gemm_f32_operands_tflops"] / PEAK_TFLOPS[args.mixed],
+            "traffic": None,
         },
         "gauge_move": {"ms": rates["gauge"]["ms"], "tflops": rates["gauge"]["tflops"],
                        "frac_fp64_peak": rates["gauge"]["tflops"] / PEAK_TFLOPS["fp64"]},
