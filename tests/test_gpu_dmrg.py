@@ -91,8 +91,10 @@ def test_parity_with_oracle_same_start(gpu):
     s_gpu = d.mps.schmidt_values()
     worst = max(np.abs(np.sort(a)[::-1] - b).max() for a, b in zip(s_gpu, s_ref))
     print(f"same-start one-site: max |ds| over bonds {worst:.2e}", flush=True)
+    # measured 1.2e-12 on MI355X: same start, same tolerance, same sweep
+    # order, so the spectra agree to rounding amplified by the solves
     for a, b in zip(s_gpu, s_ref):
-        np.testing.assert_allclose(np.sort(a)[::-1], b, atol=1e-7)
+        np.testing.assert_allclose(np.sort(a)[::-1], b, atol=1e-11)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
@@ -226,8 +228,9 @@ def test_two_site_grows_bonds_and_matches_oracle(gpu):
     pairs = list(zip(d.mps.schmidt_values(), o.schmidt_values(ref.mps)))
     worst = max(np.abs(np.sort(a)[::-1] - b).max() for a, b in pairs)
     print(f"same-start two-site: max |ds| over bonds {worst:.2e}", flush=True)
+    # measured 7e-13 on MI355X
     for a, b in pairs:
-        np.testing.assert_allclose(np.sort(a)[::-1], b, atol=1e-7)
+        np.testing.assert_allclose(np.sort(a)[::-1], b, atol=1e-11)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
@@ -251,7 +254,10 @@ def test_config2_xxz_n100_chi60_parity(gpu):
     in for primme). The two trajectories differ at the eigensolver
     tolerance and contract onto the same fixed point: the last sweep
     energies agree to 1e-9 and the per-bond Schmidt spectra of the
-    normalised states to 1e-5."""
+    normalised states to 1e-6 (measured 1.5e-7 on MI355X). The spectra are
+    bounded by the eigensolver tolerance, not by rounding: a local solve
+    stopped at residual tol * |H| leaves the eigenvector off by about
+    tol * |H| / gap, and the two solvers stop at different points."""
     import time
 
     from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
@@ -272,4 +278,4 @@ def test_config2_xxz_n100_chi60_parity(gpu):
     s_ref = o.schmidt_values(ref.mps)
     worst = max(np.abs(np.sort(a)[::-1] - b).max() for a, b in zip(s_gpu, s_ref))
     print(f"max |ds| over bonds: {worst:.2e}", flush=True)
-    assert worst < 1e-5
+    assert worst < 1e-6

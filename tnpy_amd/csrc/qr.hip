@@ -742,55 +742,55 @@ __device__ __forceinline__ void wv_lu(const double* B, double* Lf, double* U, do
   }
 }
 
-// Li = L^-1 for lower triangular L (LDS; 1/L_ii in rd, or unit diagonal
-// when rd is null): column c in lane c of the calling wave, four
-// accumulation chains per entry. Zeros above the diagonal.
-__device__ __forceinline__ void wv_lower_inverse(const double* L, const double* rd, double* Li) {
-  const int c = threadIdx.x & 63;
-  double x[kTsNB];
+// Inverse of a 32 x 32 triangular matrix in LDS (identity outside its
+// leading b x b block) by all 256 threads of the workgroup: the four
+// diagonal 8 x 8 blocks by forward substitution (one column per thread),
+// then the blocks below them by distance from the diagonal,
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj (through the LDS scratch Z), two
+// barriers per distance. The dependent chains are 8 long instead of 32.
+// TRANS: L is read transposed and X written transposed, so an upper
+// triangular U gives X = U^-1. rd: the reciprocal diagonal, or null for a
+// unit diagonal. Zeros on the other side of the diagonal.
+template <bool TRANS>
+__device__ __forceinline__ void blk_tri_inverse(const double* L, const double* rd, double* X,
+                                                double* Z) {
+  const int t = threadIdx.x;
+  auto lv = [&](int r, int c) { return TRANS ? L[c * kCqLd + r] : L[r * kCqLd + c]; };
+  auto xi = [&](int r, int c) { return TRANS ? c * kCqLd + r : r * kCqLd + c; };
+  for (int e = t; e < kCqG; e += 256) X[(e >> 5) * kCqLd + (e & 31)] = 0.0;
+  __syncthreads();
+  if (t < kTsNB) {
+    const int q8 = (t >> 3) * 8, cl = t & 7;
+    double x[8];
 #pragma unroll
-  for (int r = 0; r < kTsNB; ++r) {
-    double a0 = r == c ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int r = 0; r < 8; ++r) {
+      double acc = r == cl ? 1.0 : 0.0;
 #pragma unroll
-    for (int k = 0; k + 3 < r; k += 4) {
-      a0 = fma(-L[r * kCqLd + k], x[k], a0);
-      a1 = fma(-L[r * kCqLd + k + 1], x[k + 1], a1);
-      a2 = fma(-L[r * kCqLd + k + 2], x[k + 2], a2);
-      a3 = fma(-L[r * kCqLd + k + 3], x[k + 3], a3);
+      for (int k = 0; k < r; ++k) acc = fma(-lv(q8 + r, q8 + k), x[k], acc);
+      x[r] = r < cl ? 0.0 : (rd ? acc * rd[q8 + r] : acc);
     }
 #pragma unroll
-    for (int k = r & ~3; k < r; ++k) a0 = fma(-L[r * kCqLd + k], x[k], a0);
-    const double a = (a0 + a1) + (a2 + a3);
-    x[r] = rd ? a * rd[r] : a;
+    for (int r = 0; r < 8; ++r) X[xi(q8 + r, q8 + cl)] = x[r];
   }
-  if (c < kTsNB) {
-#pragma unroll
-    for (int r = 0; r < kTsNB; ++r) Li[r * kCqLd + c] = x[r];
-  }
-}
-
-// Ui = U^-1 for upper triangular U (LDS; 1/U_ii in rd): column c in lane c
-// of the calling wave. Zeros below the diagonal.
-__device__ __forceinline__ void wv_upper_inverse(const double* U, const double* rd, double* Ui) {
-  const int c = threadIdx.x & 63;
-  double x[kTsNB];
-#pragma unroll
-  for (int r = kTsNB - 1; r >= 0; --r) {
-    double a0 = r == c ? 1.0 : 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-    for (int k = r + 1; k + 3 < kTsNB; k += 4) {
-      a0 = fma(-U[r * kCqLd + k], x[k], a0);
-      a1 = fma(-U[r * kCqLd + k + 1], x[k + 1], a1);
-      a2 = fma(-U[r * kCqLd + k + 2], x[k + 2], a2);
-      a3 = fma(-U[r * kCqLd + k + 3], x[k + 3], a3);
+  __syncthreads();
+  for (int dist = 1; dist < 4; ++dist) {
+    const int nb = 4 - dist;  // blocks (j + dist, j), j < nb
+    for (int e = t; e < nb * 64; e += 256) {
+      const int j = e >> 6, rr = (e >> 3) & 7, cc = e & 7, i = j + dist;
+      double acc = 0.0;
+      for (int k = 8 * j; k < 8 * i; ++k) acc = fma(lv(8 * i + rr, k), X[xi(k, 8 * j + cc)], acc);
+      Z[(8 * i + rr) * kCqLd + 8 * j + cc] = acc;
     }
+    __syncthreads();
+    for (int e = t; e < nb * 64; e += 256) {
+      const int j = e >> 6, rr = (e >> 3) & 7, cc = e & 7, i = j + dist;
+      double acc = 0.0;
 #pragma unroll
-    for (int k = kTsNB - ((kTsNB - 1 - r) & 3); k < kTsNB; ++k) a0 = fma(-U[r * kCqLd + k], x[k], a0);
-    x[r] = ((a0 + a1) + (a2 + a3)) * rd[r];
-  }
-  if (c < kTsNB) {
-#pragma unroll
-    for (int r = 0; r < kTsNB; ++r) Ui[r * kCqLd + c] = x[r];
+      for (int k = 0; k < 8; ++k)
+        acc = fma(X[xi(8 * i + rr, 8 * i + k)], Z[(8 * i + k) * kCqLd + 8 * j + cc], acc);
+      X[xi(8 * i + rr, 8 * j + cc)] = -acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -905,8 +905,7 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
     if (tid == 0) { *flag = 1; *q.cnt = 0; }
     return;
   }
-  if (tid < 64) wv_lower_inverse(Lo, rd, Li);
-  __syncthreads();
+  blk_tri_inverse<false>(Lo, rd, Li, Y);  // Y is free: this block's rows are done
   if (PASS == 0) {
     cq_store(Li, q.Li);
     cq_store(Lo, q.L1s);
@@ -941,9 +940,8 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   cq_store(Li, q.Li);
   if (tid < 64) wv_lu(B, Lf, U, sg, rdU, cb);
   __syncthreads();
-  if (tid < 64) wv_lower_inverse(Lf, nullptr, Ma);  // Ma = L1f^-1
-  else if (tid < 128) wv_upper_inverse(U, rdU, G);  // G = U^-1
-  __syncthreads();
+  blk_tri_inverse<false>(Lf, nullptr, Ma, Y);  // Ma = L1f^-1
+  blk_tri_inverse<true>(U, rdU, G, Y);         // G = U^-1
   cq_store(Lf, q.L1f);
   cq_store(G, q.Ui);
   for (int e = tid; e < kCqG; e += 256) {
