@@ -85,9 +85,12 @@ def parse():
     p.add_argument("--batch-seed", type=int, default=7)
     p.add_argument("--batch-sweeps", type=int, default=2)
     p.add_argument("--only-batched", action="store_true")
-    p.add_argument("--batched-concurrent", action="store_true",
-                   help="sweep the chain-length groups concurrently (one host thread and stream each) "
-                        "instead of one after another")
+    p.add_argument("--batched-mode", choices=["grouped", "serial", "concurrent"], default="grouped",
+                   help="grouped: all length groups in lockstep, one launch per kernel per step for "
+                        "each stack of equal site shapes; serial: groups one after another; "
+                        "concurrent: groups on one host thread and stream each")
+    p.add_argument("--no-batched-compare", action="store_true",
+                   help="skip timing the per-group (serial) form beside the grouped one")
     p.add_argument("--parity", action="store_true",
                    help="also run the converged mixed-vs-FP64 comparison from a random state at chi "
                         "(n=--parity-n, --parity-sweeps sweeps each)")
@@ -495,10 +498,15 @@ def batched_leg(args, dev, budget=None):
     models = [RandomCouplingHeisenberg(n=n, seed=sd) for n, sd in zip(ns, cseeds)]
     chi = args.batch_chi
     t = time.perf_counter()
-    bd = BatchedFiniteDMRG([m.mpo for m in models], chi=chi,
-                           seeds=[args.batch_seed * 100003 + i for i in range(len(ns))],
-                           max_basis=args.max_basis, keep=args.keep, concurrent=args.batched_concurrent,
-                           device=dev)
+    seeds = [args.batch_seed * 100003 + i for i in range(len(ns))]
+    mode = args.batched_mode
+
+    def make(m):
+        return BatchedFiniteDMRG([x.mpo for x in models], chi=chi, seeds=seeds, max_basis=args.max_basis,
+                                 keep=args.keep, grouped=m == "grouped", concurrent=m == "concurrent",
+                                 device=dev)
+
+    bd = make(mode)
     torch.cuda.synchronize()
     log(f"batched: {len(ns)} chains {dict(sorted((n, ns.count(n)) for n in set(ns)))} chi={chi} "
         f"ready in {time.perf_counter() - t:.1f}s")
@@ -517,6 +525,28 @@ def batched_leg(args, dev, budget=None):
             sweeps = k + 1
             break
     dt = time.perf_counter() - t0
+    solves = bd.stats["dense_sites"] + bd.stats["krylov_sites"]
+    compare = None
+    if mode == "grouped" and not args.no_batched_compare and (budget is None or budget.left() > 2.5 * dt):
+        # the same sweeps with the length groups one after another
+        bs = make("serial")
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        es = [bs.sweep(directions[k % 2], tol=args.tol) for k in range(sweeps)]
+        torch.cuda.synchronize()
+        ds = time.perf_counter() - ts
+        compare = {
+            "mode": "serial",
+            "seconds": ds,
+            "ms_per_step": 1e3 * ds / sweeps,
+            "solve_launch_sets": bs.stats["dense_sites"] + bs.stats["krylov_sites"],
+            "lockstep_matvecs": bs.stats["lockstep_matvecs"],
+            "grouped_speedup": ds / dt,
+            "max_rel_dE_vs_grouped": float(np.max(np.abs(es[-1] - eb[-1]) / np.abs(eb[-1]))),
+        }
+        log(f"batched per-group form: {sweeps} sweep(s) in {ds:.2f}s ({compare['solve_launch_sets']} solve "
+            f"launch sets against {solves} grouped); grouped {ds / dt:.2f}x faster; max rel |dE| "
+            f"{compare['max_rel_dE_vs_grouped']:.2e}")
     single = {}
     dEmax = 0.0
     for i in sample:
@@ -537,7 +567,8 @@ def batched_leg(args, dev, budget=None):
         "chains": len(ns),
         "lengths": {str(n): ns.count(n) for n in sorted(set(ns))},
         "sweeps": sweeps,
-        "concurrent_groups": args.batched_concurrent,
+        "mode": mode,
+        "solve_launch_sets": solves,
         "seconds": dt,
         "sweeps_per_s": sweeps / dt,
         "chain_sweeps_per_s": len(ns) * sweeps / dt,
@@ -547,6 +578,7 @@ def batched_leg(args, dev, budget=None):
         "sample_rel_dE_vs_unbatched": dEmax,
         "matvecs": bd.stats["matvecs"],
         "lockstep_matvecs": bd.stats["lockstep_matvecs"],
+        "per_group_form": compare,
     }
 
 

@@ -287,11 +287,12 @@ def _chains(ns, seed):
     return [RandomCouplingHeisenberg(n=n, seed=int(rng.integers(1 << 30))) for n in ns]
 
 
-@pytest.mark.parametrize("concurrent", [False, True])
-def test_batched_dmrg_matches_single(gpu, concurrent):
+@pytest.mark.parametrize("mode", ["grouped", "serial", "concurrent"])
+def test_batched_dmrg_matches_single(gpu, mode):
     """Per chain, sweep by sweep, the batched driver equals FiniteDMRG from
-    the same initial state (mixed lengths, dense and Krylov sites), with the
-    length groups swept one after another and concurrently."""
+    the same initial state (mixed lengths, dense and Krylov sites): length
+    groups stepped in lockstep with shared launches, swept one after
+    another, and swept concurrently."""
     from tnpy_amd import Direction, FiniteDMRG, MatrixProductState
     from tnpy_amd.batched_dmrg import BatchedFiniteDMRG
 
@@ -300,10 +301,20 @@ def test_batched_dmrg_matches_single(gpu, concurrent):
     models = _chains(ns, 7)
     starts = [MatrixProductState.random(n, chi, 2, seed=100 + i, device=gpu) for i, n in enumerate(ns)]
     bd = BatchedFiniteDMRG([m.mpo for m in models], bond_dim=chi,
-                           mps=[s.copy() for s in starts], device=gpu, concurrent=concurrent)
+                           mps=[s.copy() for s in starts], device=gpu, grouped=mode == "grouped",
+                           concurrent=mode == "concurrent")
     assert bd.groups == {10: [0, 2], 12: [4], 14: [1, 3]}
     dirs = [Direction.RIGHTWARD, Direction.LEFTWARD, Direction.RIGHTWARD]
     eb = np.array([bd.sweep(dr, tol=1e-12) for dr in dirs]).T
+    solves = bd.stats["dense_sites"] + bd.stats["krylov_sites"]
+    per_group = len(dirs) * sum(n - 1 for n in (10, 12, 14))
+    print(f"{mode}: {solves} solve launches for {per_group} group site steps", flush=True)
+    if mode == "grouped":
+        # at least one step per site of the longest chain, and fewer solve
+        # launches than group site steps (shared stacks)
+        assert len(dirs) * 13 <= solves < per_group
+    else:
+        assert solves == per_group
     for i, m in enumerate(models):
         f = FiniteDMRG(m.mpo, bond_dim=chi, mps=starts[i].copy(), compute_variance=False, device=gpu)
         es = [f.sweep(dr, tol=1e-12) for dr in dirs]

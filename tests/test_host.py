@@ -150,3 +150,23 @@ class TestABI:
 
         with pytest.raises(RuntimeError, match="GPU"):
             FiniteDMRG(XXZ(n=6, delta=0.5).mpo, bond_dim=4)
+
+
+def test_batched_stack_joins_adjacent_views_without_copy():
+    """The grouped batched sweep joins the stacks of several length groups;
+    stacks that lie back to back in one buffer (the previous step's output)
+    are joined as a view, anything else by a copy in the given order."""
+    import torch
+
+    from tnpy_amd.batched_dmrg import _stack
+
+    buf = torch.arange(5 * 2 * 3, dtype=torch.float64).reshape(5, 2, 3)
+    a, b = buf[0:2], buf[2:5]
+    v = _stack([a, b])
+    assert v.data_ptr() == buf.data_ptr() and torch.equal(v, buf)
+    other = torch.zeros(1, 2, 3, dtype=torch.float64)
+    w = _stack([a, other])
+    assert w.data_ptr() != buf.data_ptr() and torch.equal(w, torch.cat([a, other]))
+    x = _stack([b, a])  # out of order: copied
+    assert x.data_ptr() != buf.data_ptr() and torch.equal(x, torch.cat([b, a]))
+    assert _stack([a]) is a

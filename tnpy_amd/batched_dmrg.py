@@ -15,7 +15,11 @@ each site step for the whole group with one launch per kernel:
 - the QR / LQ gauge move (one workgroup per chain, LDS resident),
 - the environment update.
 
-Per chain the result equals a ``FiniteDMRG`` run from the same initial
+Groups of different lengths step in lockstep (rightward sweeps aligned at
+the left end, leftward sweeps at the right end), and the groups whose
+current sites have equal shapes are joined into one stack, so a step is one
+launch per kernel for all of them; away from the right ends every group has
+the same interior shape. Per chain the result equals a ``FiniteDMRG`` run from the same initial
 state up to rounding. FP64 throughout.
 """
 from __future__ import annotations
@@ -34,6 +38,163 @@ import torch
 from tnpy_amd import _lib, logger
 from tnpy_amd.matrix_product_state import Direction, MatrixProductState, _bond_dims
 from tnpy_amd.operators import as_mpo
+
+
+def _env_left(ws, stream, L, A, W, dims, B) -> torch.Tensor:
+    """Stacked left environments of the next site (tnx_env_left_batched)."""
+    l, Dl, Dr, d, r = dims
+    out = torch.empty(B, r, Dr, r, dtype=torch.float64, device=A.device)
+    lib = _lib.lib()
+    wp, wb = ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, B))
+    _lib.check(
+        lib.tnx_env_left_batched(
+            _lib.ptr(L, "L"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Lout"),
+            l, Dl, Dr, d, r, B, _lib.TNX_F64, wp, wb, stream,
+        ),
+        "tnx_env_left_batched",
+    )
+    return out
+
+
+def _env_right(ws, stream, R, A, W, dims, B) -> torch.Tensor:
+    """Stacked right environments of the previous site (tnx_env_right_batched)."""
+    l, Dl, Dr, d, r = dims
+    out = torch.empty(B, l, Dl, l, dtype=torch.float64, device=A.device)
+    lib = _lib.lib()
+    wp, wb = ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, B))
+    _lib.check(
+        lib.tnx_env_right_batched(
+            _lib.ptr(R, "R"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Rout"),
+            l, Dl, Dr, d, r, B, _lib.TNX_F64, wp, wb, stream,
+        ),
+        "tnx_env_right_batched",
+    )
+    return out
+
+
+def _matvec(ws, stream, L, W, R, x, y, dims, B) -> None:
+    l, Dl, Dr, d, r = dims
+    lib = _lib.lib()
+    wp, wb = ws.get(lib.tnx_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, B))
+    _lib.check(
+        lib.tnx_heff1_matvec_batched(
+            _lib.ptr(L, "L"), _lib.ptr(W, "W"), _lib.ptr(R, "R"), _lib.ptr(x, "x"), _lib.ptr(y, "y"),
+            l, Dl, Dr, d, r, B, _lib.TNX_F64, wp, wb, stream,
+        ),
+        "tnx_heff1_matvec_batched",
+    )
+
+
+def _solve(owner, ws, stream, L, W, R, v0, dims, B, chains, site, tol):
+    """Lowest eigenpair of each of B stacked H_eff: (energies, vec, hvec),
+    vec/hvec stacked (B, N). ``chains`` names the problems in warnings."""
+    lib = _lib.lib()
+    l, Dl, Dr, d, r = dims
+    N = l * d * r
+    dev = owner.device
+    vec = torch.empty(B, N, dtype=torch.float64, device=dev)
+    hvec = torch.empty(B, N, dtype=torch.float64, device=dev)
+    evals = (C.c_double * B)()
+    if N < owner._exact_solver_dim:
+        M = torch.empty(B, N, N, dtype=torch.float64, device=dev)
+        _lib.check(
+            lib.tnx_heff1_dense_batched(
+                _lib.ptr(L, "L"), _lib.ptr(W, "W"), _lib.ptr(R, "R"), _lib.ptr(M, "M"),
+                l, Dl, Dr, d, r, B, stream,
+            ),
+            "tnx_heff1_dense_batched",
+        )
+        wp, wb = ws.get(lib.tnx_syev_lowest_batched_workspace_bytes(N, B))
+        _lib.check(
+            lib.tnx_syev_lowest_batched(_lib.ptr(M, "M"), N, B, _lib.ptr(vec, "evec"), evals, wp, wb, stream),
+            "tnx_syev_lowest_batched",
+        )
+        _matvec(ws, stream, L, W, R, vec, hvec, dims, B)
+        with owner._lock:
+            owner.stats["dense_sites"] += 1
+        return np.array(evals[:]), vec, hvec
+    opts = _lib.EigshOpts()
+    opts.tol = float(tol)
+    opts.max_basis = int(owner._max_basis)
+    opts.max_matvecs = int(owner._max_matvecs)
+    opts.precision = _lib.TNX_F64
+    opts.keep = int(owner._keep)
+    infos = (_lib.EigshInfo * B)()
+    wp, wb = ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, B))
+    v0 = v0.reshape(B, N)
+    st = lib.tnx_eigsh_heff1_batched(
+        _lib.ptr(L, "L"), _lib.ptr(W, "W"), _lib.ptr(R, "R"), _lib.ptr(v0, "v0"), _lib.ptr(vec, "vec"),
+        _lib.ptr(hvec, "hvec"), l, Dl, Dr, d, r, B, C.byref(opts), infos, wp, wb, stream,
+    )
+    if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
+        _lib.check(st, "tnx_eigsh_heff1_batched")
+    if st == _lib.TNX_ERR_NOCONV:
+        bad = [chains[i] for i in range(B) if not infos[i].converged]
+        logger.warning(f"Lanczos at site {site} hit the matvec limit for chains {bad}.")
+    with owner._lock:
+        owner.stats["krylov_sites"] += 1
+        owner.stats["matvecs"] += sum(infos[i].matvecs for i in range(B))
+        owner.stats["lockstep_matvecs"] += max(infos[i].matvecs for i in range(B))
+        owner.stats["restarts"] += max(infos[i].restarts for i in range(B))
+    return np.array([infos[i].eval for i in range(B)]), vec, hvec
+
+
+def _gauge_right(ws, stream, A, nxt, B) -> torch.Tensor:
+    """QR of each stacked site tensor A (in place) with R absorbed into the
+    next site; returns the new next-site stack."""
+    lib = _lib.lib()
+    _, l, p, r = A.shape
+    out = torch.empty_like(nxt)
+    wp, wb = ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, B))
+    _lib.check(
+        lib.tnx_gauge_right_batched(
+            _lib.ptr(A, "site"), _lib.ptr(nxt, "next"), _lib.ptr(out, "next_out"),
+            l, p, r, nxt.shape[2] * nxt.shape[3], B, wp, wb, stream,
+        ),
+        "tnx_gauge_right_batched",
+    )
+    return out
+
+
+def _gauge_left(ws, stream, A, prv, B) -> torch.Tensor:
+    """LQ of each stacked site tensor A (in place) with L absorbed into the
+    previous site; returns the new previous-site stack."""
+    lib = _lib.lib()
+    _, l, p, r = A.shape
+    out = torch.empty_like(prv)
+    wp, wb = ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, B))
+    _lib.check(
+        lib.tnx_gauge_left_batched(
+            _lib.ptr(A, "site"), _lib.ptr(prv, "prev"), _lib.ptr(out, "prev_out"),
+            l, p, r, prv.shape[1] * prv.shape[2], B, wp, wb, stream,
+        ),
+        "tnx_gauge_left_batched",
+    )
+    return out
+
+
+def _stack(parts: List[torch.Tensor]) -> torch.Tensor:
+    """The stacks of several groups as one stack (chain index first). When
+    the parts already lie back to back in one buffer (the outputs of the
+    previous merged step) this is a view of that buffer, else a copy."""
+    if len(parts) == 1:
+        return parts[0]
+    first = parts[0]
+    rest = first.shape[1:]
+    ok = all(p.is_contiguous() and p.shape[1:] == rest for p in parts)
+    if ok:
+        st = first.untyped_storage()
+        ptr = first.data_ptr()
+        for p in parts:
+            if p.untyped_storage().data_ptr() != st.data_ptr() or p.data_ptr() != ptr:
+                ok = False
+                break
+            ptr += p.numel() * p.element_size()
+    if ok:
+        total = sum(p.shape[0] for p in parts)
+        return first.new_empty(0).set_(st, first.storage_offset(), (total,) + tuple(rest),
+                                       first.stride())
+    return torch.cat(parts)
 
 
 class _Group:
@@ -73,34 +234,12 @@ class _Group:
         return l, int(W.shape[1]), int(W.shape[2]), d, r
 
     def update_left(self, site: int) -> None:
-        L, A, W = self.left[site - 1], self.A[site - 1], self.W[site - 1]
-        l, Dl, Dr, d, r = self.dims(site - 1)
-        out = torch.empty(self.B, r, Dr, r, dtype=torch.float64, device=A.device)
-        lib = _lib.lib()
-        wp, wb = self.ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
-        _lib.check(
-            lib.tnx_env_left_batched(
-                _lib.ptr(L, "L"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Lout"),
-                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.stream.cuda_stream,
-            ),
-            "tnx_env_left_batched",
-        )
-        self.left[site] = out
+        self.left[site] = _env_left(self.ws, self.stream.cuda_stream, self.left[site - 1], self.A[site - 1],
+                                    self.W[site - 1], self.dims(site - 1), self.B)
 
     def update_right(self, site: int) -> None:
-        R, A, W = self.right[site + 1], self.A[site + 1], self.W[site + 1]
-        l, Dl, Dr, d, r = self.dims(site + 1)
-        out = torch.empty(self.B, l, Dl, l, dtype=torch.float64, device=A.device)
-        lib = _lib.lib()
-        wp, wb = self.ws.get(lib.tnx_env_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
-        _lib.check(
-            lib.tnx_env_right_batched(
-                _lib.ptr(R, "R"), _lib.ptr(A, "A"), _lib.ptr(W, "W"), _lib.ptr(out, "Rout"),
-                l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb, self.stream.cuda_stream,
-            ),
-            "tnx_env_right_batched",
-        )
-        self.right[site] = out
+        self.right[site] = _env_right(self.ws, self.stream.cuda_stream, self.right[site + 1], self.A[site + 1],
+                                      self.W[site + 1], self.dims(site + 1), self.B)
 
     def ensure_left(self, site: int) -> None:
         first = max(k for k in self.left if k <= site)
@@ -108,101 +247,20 @@ class _Group:
             self.update_left(s)
 
     def matvec(self, site: int, x: torch.Tensor, y: torch.Tensor) -> None:
-        l, Dl, Dr, d, r = self.dims(site)
-        lib = _lib.lib()
-        wp, wb = self.ws.get(lib.tnx_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, self.B))
-        _lib.check(
-            lib.tnx_heff1_matvec_batched(
-                _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
-                _lib.ptr(x, "x"), _lib.ptr(y, "y"), l, Dl, Dr, d, r, self.B, _lib.TNX_F64, wp, wb,
-                self.stream.cuda_stream,
-            ),
-            "tnx_heff1_matvec_batched",
-        )
+        _matvec(self.ws, self.stream.cuda_stream, self.left[site], self.W[site], self.right[site], x, y,
+                self.dims(site), self.B)
 
     def solve(self, site: int, tol: float):
         """Lowest eigenpair of every chain's H_eff at ``site``: (energies,
         vec, hvec), vec/hvec stacked (B, N)."""
-        o = self.owner
-        lib = _lib.lib()
-        l, Dl, Dr, d, r = self.dims(site)
-        N = l * d * r
-        dev = o.device
-        vec = torch.empty(self.B, N, dtype=torch.float64, device=dev)
-        hvec = torch.empty(self.B, N, dtype=torch.float64, device=dev)
-        evals = (C.c_double * self.B)()
-        if N < o._exact_solver_dim:
-            M = torch.empty(self.B, N, N, dtype=torch.float64, device=dev)
-            _lib.check(
-                lib.tnx_heff1_dense_batched(
-                    _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
-                    _lib.ptr(M, "M"), l, Dl, Dr, d, r, self.B, self.stream.cuda_stream,
-                ),
-                "tnx_heff1_dense_batched",
-            )
-            wp, wb = self.ws.get(lib.tnx_syev_lowest_batched_workspace_bytes(N, self.B))
-            _lib.check(
-                lib.tnx_syev_lowest_batched(_lib.ptr(M, "M"), N, self.B, _lib.ptr(vec, "evec"), evals, wp, wb,
-                                            self.stream.cuda_stream),
-                "tnx_syev_lowest_batched",
-            )
-            self.matvec(site, vec, hvec)
-            with o._lock:
-                o.stats["dense_sites"] += 1
-            return np.array(evals[:]), vec, hvec
-        opts = _lib.EigshOpts()
-        opts.tol = float(tol)
-        opts.max_basis = int(o._max_basis)
-        opts.max_matvecs = int(o._max_matvecs)
-        opts.precision = _lib.TNX_F64
-        opts.keep = int(o._keep)
-        infos = (_lib.EigshInfo * self.B)()
-        wp, wb = self.ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, self.B))
-        v0 = self.A[site].reshape(self.B, N)
-        st = lib.tnx_eigsh_heff1_batched(
-            _lib.ptr(self.left[site], "L"), _lib.ptr(self.W[site], "W"), _lib.ptr(self.right[site], "R"),
-            _lib.ptr(v0, "v0"), _lib.ptr(vec, "vec"), _lib.ptr(hvec, "hvec"), l, Dl, Dr, d, r, self.B,
-            C.byref(opts), infos, wp, wb, self.stream.cuda_stream,
-        )
-        if st not in (_lib.TNX_OK, _lib.TNX_ERR_NOCONV):
-            _lib.check(st, "tnx_eigsh_heff1_batched")
-        if st == _lib.TNX_ERR_NOCONV:
-            bad = [self.chains[i] for i in range(self.B) if not infos[i].converged]
-            logger.warning(f"Lanczos at site {site} hit the matvec limit for chains {bad}.")
-        with o._lock:
-            o.stats["krylov_sites"] += 1
-            o.stats["matvecs"] += sum(infos[i].matvecs for i in range(self.B))
-            o.stats["lockstep_matvecs"] += max(infos[i].matvecs for i in range(self.B))
-            o.stats["restarts"] += max(infos[i].restarts for i in range(self.B))
-        return np.array([infos[i].eval for i in range(self.B)]), vec, hvec
+        return _solve(self.owner, self.ws, self.stream.cuda_stream, self.left[site], self.W[site],
+                      self.right[site], self.A[site], self.dims(site), self.B, self.chains, site, tol)
 
     def gauge(self, site: int, direction: Direction) -> None:
-        lib = _lib.lib()
-        A = self.A[site]
-        _, l, p, r = A.shape
-        wp, wb = self.ws.get(lib.tnx_gauge_batched_workspace_bytes(l, p, r, self.B))
         if direction == Direction.RIGHTWARD:
-            nxt = self.A[site + 1]
-            out = torch.empty_like(nxt)
-            _lib.check(
-                lib.tnx_gauge_right_batched(
-                    _lib.ptr(A, "site"), _lib.ptr(nxt, "next"), _lib.ptr(out, "next_out"),
-                    l, p, r, nxt.shape[2] * nxt.shape[3], self.B, wp, wb, self.stream.cuda_stream,
-                ),
-                "tnx_gauge_right_batched",
-            )
-            self.A[site + 1] = out
+            self.A[site + 1] = _gauge_right(self.ws, self.stream.cuda_stream, self.A[site], self.A[site + 1], self.B)
         else:
-            prv = self.A[site - 1]
-            out = torch.empty_like(prv)
-            _lib.check(
-                lib.tnx_gauge_left_batched(
-                    _lib.ptr(A, "site"), _lib.ptr(prv, "prev"), _lib.ptr(out, "prev_out"),
-                    l, p, r, prv.shape[1] * prv.shape[2], self.B, wp, wb, self.stream.cuda_stream,
-                ),
-                "tnx_gauge_left_batched",
-            )
-            self.A[site - 1] = out
+            self.A[site - 1] = _gauge_left(self.ws, self.stream.cuda_stream, self.A[site], self.A[site - 1], self.B)
 
     # -- site step and sweep -------------------------------------------------------
     def site_step(self, site: int, direction: Direction, tol: float) -> np.ndarray:
@@ -249,11 +307,17 @@ class BatchedFiniteDMRG:
             each drawn as ``MatrixProductState.random(n, chi, d, seed)``.
         exact_solver_dim, max_basis, keep: as for ``FiniteDMRG``.
         max_matvecs: Lanczos matvec limit per site.
-        concurrent: sweep the length groups concurrently (one host thread
-            and stream each). Off by default: the groups run one after
-            another (the same kernels), because the concurrent form has
-            ended in a host fault inside the launch path under a kernel
-            tracer, a cause not yet found.
+        grouped: sweep all length groups in lockstep with one launch per
+            kernel per step for all chains whose current site has the same
+            shape (default). Rightward sweeps align the groups at the left
+            end, leftward sweeps at the right end, so the site shapes agree
+            everywhere except within log2(chi) sites of the far end of the
+            shorter chains; a step takes one launch set per distinct shape.
+        concurrent: with ``grouped=False``, sweep the length groups
+            concurrently (one host thread and stream each) instead of one
+            after another. Off by default: that form has ended in a host
+            fault inside the launch path under a kernel tracer, a cause not
+            yet found.
     """
 
     def __init__(
@@ -268,6 +332,7 @@ class BatchedFiniteDMRG:
         max_basis: int = 64,
         keep: int = 20,
         max_matvecs: int = 5000,
+        grouped: bool = True,
         concurrent: bool = False,
         device: Optional[torch.device] = None,
     ):
@@ -290,6 +355,7 @@ class BatchedFiniteDMRG:
         self._max_basis = max_basis
         self._keep = keep
         self._max_matvecs = max_matvecs
+        self._grouped = grouped
         self._concurrent = concurrent
         self._ws = _lib.Workspace(self.device)
         self._lock = threading.Lock()
@@ -319,6 +385,8 @@ class BatchedFiniteDMRG:
                     raise ValueError(f"chains of length {n} have different bond dimensions")
                 stacks = [torch.stack([s[k] for s in starts]).contiguous() for k in range(n)]
             self._groups.append(_Group(self, idx, [self._mpos[i] for i in idx], stacks))
+        # the stream of grouped sweeps (default priority)
+        self._gstream = torch.cuda.Stream(device=self.device)
         torch.cuda.synchronize(self.device)
         self._energies: List[List[float]] = [[] for _ in range(nchain)]
         self.sweep_times: List[float] = []
@@ -358,10 +426,13 @@ class BatchedFiniteDMRG:
 
     def sweep(self, direction: Direction = Direction.RIGHTWARD, tol: float = 1e-8) -> np.ndarray:
         """One directional sweep of every chain; returns each chain's
-        energy at the last site of the sweep. Length groups sweep
-        concurrently, one host thread and one stream each (the library
-        calls release the GIL), so one group's small, latency-bound launches
-        overlap another's."""
+        energy at the last site of the sweep. By default all length groups
+        step in lockstep, one launch per kernel per step for every stack of
+        equal site shapes; with ``grouped=False`` the groups sweep one after
+        another or, with ``concurrent=True``, concurrently on one host
+        thread and stream each (the library calls release the GIL)."""
+        if self._grouped and len(self._groups) > 1:
+            return self._sweep_grouped(direction, tol)
         out = np.empty(self.n_chains)
         if len(self._groups) == 1 or not self._concurrent:
             for g in self._groups:
@@ -372,6 +443,73 @@ class BatchedFiniteDMRG:
         for g, f in futs:
             out[g.chains] = f.result()
         return out
+
+    def _sweep_grouped(self, direction: Direction, tol: float) -> np.ndarray:
+        """One directional sweep of all groups in lockstep: at step t each
+        group with t < n - 1 is at site t (rightward) or n - 1 - t
+        (leftward); the groups whose site and neighbour shapes agree form one
+        stack, and each stack takes one launch per kernel for its solve,
+        psi += 1e-5 H psi, gauge move and environment update. Per chain the
+        arithmetic is that of the per-group sweep."""
+        gs_all = self._groups
+        ms = self._gstream
+        right = direction == Direction.RIGHTWARD
+        for g in gs_all:
+            if not right:
+                with torch.cuda.stream(g.stream):
+                    g.ensure_left(g.n - 1)
+            ms.wait_stream(g.stream)
+        out = np.empty(self.n_chains)
+        sp = ms.cuda_stream
+        with torch.cuda.stream(ms):
+            for t in range(max(g.n for g in gs_all) - 1):
+                stacks: Dict[tuple, list] = {}
+                for g in gs_all:
+                    if t >= g.n - 1:
+                        continue
+                    site = t if right else g.n - 1 - t
+                    nb = g.A[site + 1] if right else g.A[site - 1]
+                    key = (g.dims(site), tuple(nb.shape[1:]))
+                    stacks.setdefault(key, []).append((g, site))
+                for (dims, _), members in stacks.items():
+                    self._grouped_step(members, dims, right, tol, sp, out)
+            ms.synchronize()
+        return out
+
+    def _grouped_step(self, members, dims, right: bool, tol: float, sp: int, out: np.ndarray) -> None:
+        ws = self._ws
+        B = sum(g.B for g, _ in members)
+        L = _stack([g.left[s] for g, s in members])
+        R = _stack([g.right[s] for g, s in members])
+        W = _stack([g.W[s] for g, s in members])
+        A0 = _stack([g.A[s] for g, s in members])
+        chains = [c for g, _ in members for c in g.chains]
+        sites = sorted({s for _, s in members})
+        e, vec, hvec = _solve(self, ws, sp, L, W, R, A0, dims, B, chains, sites, tol)
+        _lib.check(_lib.lib().tnx_axpy(vec.numel(), 1e-5, _lib.ptr(hvec, "hvec"), _lib.ptr(vec, "vec"), sp),
+                   "tnx_axpy")
+        A = vec.reshape(A0.shape)
+        if right:
+            nb = _gauge_right(ws, sp, A, _stack([g.A[s + 1] for g, s in members]), B)
+            env = _env_left(ws, sp, L, A, W, dims, B)
+        else:
+            nb = _gauge_left(ws, sp, A, _stack([g.A[s - 1] for g, s in members]), B)
+            env = _env_right(ws, sp, R, A, W, dims, B)
+        off = 0
+        for g, s in members:
+            sl = slice(off, off + g.B)
+            off += g.B
+            g.A[s] = A[sl]
+            if right:
+                g.A[s + 1] = nb[sl]
+                g.left[s + 1] = env[sl]
+                last = s == g.n - 2
+            else:
+                g.A[s - 1] = nb[sl]
+                g.right[s - 1] = env[sl]
+                last = s == 1
+            if last:
+                out[g.chains] = e[sl]
 
     def run(self, tol: float = 1e-8, max_sweep: int = 100) -> List[List[float]]:
         """Sweep all chains (alternating directions) until every chain's
