@@ -16,6 +16,7 @@
 #include "tnx_common.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace tnx {
 
@@ -605,15 +606,6 @@ struct CqBuf {
   int* cnt;      // arrival ticket, zero between kernels
 };
 
-// Lane i's value of a wave-distributed double as a wave-uniform scalar
-// (v_readlane); i must be wave-uniform.
-__device__ __forceinline__ double qr_lane_val(double v, int i) {
-  const long long u = __double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), i);
-  const unsigned hi = __builtin_amdgcn_readlane((int)(u >> 32), i);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // True in the workgroup that finishes last; its reads then see every other
 // workgroup's writes made before their arrival.
 __device__ __forceinline__ bool cq_arrive_last(int* cnt) {
@@ -626,42 +618,135 @@ __device__ __forceinline__ bool cq_arrive_last(int* cnt) {
   return last;
 }
 
-// G = sum of nblk partial Grams, lower triangle, fixed order per entry, 16
-// loads in flight per thread; entries outside b x b are the identity's.
+// G = sum of nblk partial Grams, lower triangle, in the fixed order k = 0,
+// 1, ... per entry; the 528 lower entries are spread over the threads
+// (compact index p = i (i + 1) / 2 + c) with 16 partials in flight per
+// entry. Entries outside b x b are the identity's, the upper triangle zero.
 __device__ __forceinline__ void cq_gram_reduce(const double* __restrict__ Gp, int nblk, int b, double* G) {
-  int e[4];
-  bool on[4];
-  double s[4];
+  constexpr int kLow = kTsNB * (kTsNB + 1) / 2;  // 528
+  int e[3];
+  bool on[3];
+  double s[3];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    e[q] = threadIdx.x + 256 * q;
-    const int i = e[q] >> 5, c = e[q] & 31;
-    on[q] = c <= i && i < b;
+  for (int q = 0; q < 3; ++q) {
+    const int p = threadIdx.x + 256 * q;
+    int i = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+    if ((i + 1) * (i + 2) / 2 <= p) ++i;
+    if (i * (i + 1) / 2 > p) --i;
+    const int c = p - i * (i + 1) / 2;
+    e[q] = p < kLow ? i * kTsNB + c : -1;
+    on[q] = p < kLow && i < b;
     s[q] = 0.0;
   }
-  for (int k = 0; k < nblk; k += 4) {
-    double v[4][4];
+  for (int k = 0; k < nblk; k += 16) {
+    double v[3][16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        v[u][q] = on[q] && k + u < nblk ? Gp[(size_t)(k + u) * kCqG + e[q]] : 0.0;
+      for (int u = 0; u < 16; ++u)
+        v[q][u] = on[q] && k + u < nblk ? Gp[(size_t)(k + u) * kCqG + e[q]] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) s[q] += v[u][q];
+      for (int u = 0; u < 16; ++u) s[q] += v[q][u];
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 3; ++q) {
+    if (e[q] < 0) continue;
     const int i = e[q] >> 5, c = e[q] & 31;
     G[i * kCqLd + c] = on[q] ? s[q] : (i == c ? 1.0 : 0.0);
+  }
+  for (int x = threadIdx.x; x < kCqG; x += 256) {
+    const int i = x >> 5, c = x & 31;
+    if (c > i) G[i * kCqLd + c] = 0.0;
   }
   __syncthreads();
 }
 
-// Orders one wave's LDS write before its later reads of other lanes' values
-// (LDS serves a wave's instructions in order; this keeps the compiler from
-// moving them).
+// LU of B - S without pivoting (B in LDS; S_jj = -sign of the pivot, so
+// |U_jj| >= 1): the unit lower factor to Lf, U to U (zeros elsewhere), S to
+// sg, 1/U_jj to rdU. By all 256 threads of the workgroup, right looking:
+// thread t keeps row t / 8, columns 4 (t % 8) .. + 3 in registers; the
+// pivot row and pivot column of step j are published in LDS by the threads
+// that own them, double-buffered by the step's parity, with one barrier per
+// step. (Measured faster than one wave with a row per lane, whose pivot row
+// leaves a single lane one value at a time; for the Cholesky below, whose
+// column is published by all lanes at once, the one-wave form is faster.)
+// pub: 4 * kTsNB doubles, 16-byte aligned.
+
+typedef double qr_f64x2 __attribute__((ext_vector_type(2)));
+
+// entry q of a thread's four registers, q known only at run time
+__device__ __forceinline__ double pick4(const double (&a)[4], int q) {
+  return q == 0 ? a[0] : (q == 1 ? a[1] : (q == 2 ? a[2] : a[3]));
+}
+
+__device__ __forceinline__ void blk_lu(const double* B, double* Lf, double* U, double* sg,
+                                       double* rdU, double* pub) {
+  const int t = threadIdx.x, i = t >> 3, c0 = (t & 7) * 4;
+  double a[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = B[i * kCqLd + c0 + q];
+  double* rowb = pub;               // [2][kTsNB]: row j of the step
+  double* colb = pub + 2 * kTsNB;   // [2][kTsNB]: column j of the step
+  if (i == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rowb[c0 + q] = a[q];
+  }
+  if (c0 == 0) colb[i] = a[0];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTsNB; ++j) {
+    const double* rb = rowb + (j & 1) * kTsNB;
+    const double* cl = colb + (j & 1) * kTsNB;
+    const double av = rb[j];
+    const double sgn = av >= 0.0 ? -1.0 : 1.0;
+    const double piv = av - sgn;
+    const double rp = rcp_nr(piv);
+    const double l = i > j ? cl[i] * rp : 0.0;
+    const qr_f64x2 u0 = *reinterpret_cast<const qr_f64x2*>(rb + c0);
+    const qr_f64x2 u1 = *reinterpret_cast<const qr_f64x2*>(rb + c0 + 2);
+    const double uc[4] = {u0[0], u0[1], u1[0], u1[1]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c0 + q;
+      if (i > j && c > j) a[q] = fma(-l, uc[q], a[q]);
+      else if (i > j && c == j) a[q] = l;
+      else if (i == j && c == j) a[q] = piv;
+    }
+    if (j + 1 < kTsNB) {
+      double* rn = rowb + ((j + 1) & 1) * kTsNB;
+      double* cn = colb + ((j + 1) & 1) * kTsNB;
+      if (i == j + 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rn[c0 + q] = a[q];
+      }
+      const int qn = j + 1 - c0;
+      if (i > j + 1 && qn >= 0 && qn < 4) cn[i] = pick4(a, qn);
+    }
+    if (t == 0) { sg[j] = sgn; rdU[j] = rp; }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int c = c0 + q;
+    Lf[i * kCqLd + c] = c < i ? a[q] : (c == i ? 1.0 : 0.0);
+    U[i * kCqLd + c] = c >= i ? a[q] : 0.0;
+  }
+}
+
+// Lane i's value of a wave-distributed double as a wave-uniform scalar
+// (v_readlane); i must be wave-uniform.
+__device__ __forceinline__ double qr_lane_val(double v, int i) {
+  const long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), i);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(u >> 32), i);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Orders one wave's LDS writes before its later reads of other lanes'
+// values (LDS serves a wave's instructions in order; this keeps the
+// compiler from moving them).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -700,46 +785,6 @@ __device__ __forceinline__ bool wv_cholesky(const double* G, double* Lo, double*
     for (int c = 0; c < kTsNB; ++c) Lo[i * kCqLd + c] = c <= i ? g[c] : 0.0;
   }
   return ok;
-}
-
-// LU of B - S without pivoting (B in LDS), S_jj = -sign of the pivot so
-// |U_jj| >= 1, by the calling wave with row i in lane i; row j of U goes
-// to the LDS buffer cb for the other lanes. The unit lower factor to Lf, U
-// to U (zeros elsewhere), S to sg, 1/U_jj to rdU.
-__device__ __forceinline__ void wv_lu(const double* B, double* Lf, double* U, double* sg, double* rdU, double* cb) {
-  const int lane = threadIdx.x & 63;
-  const int i = lane < kTsNB ? lane : kTsNB - 1;
-  double g[kTsNB];
-#pragma unroll
-  for (int c = 0; c < kTsNB; ++c) g[c] = B[i * kCqLd + c];
-#pragma unroll
-  for (int j = 0; j < kTsNB; ++j) {
-    const double a = qr_lane_val(g[j], j);
-    const double sgn = a >= 0.0 ? -1.0 : 1.0;
-    const double piv = a - sgn;
-    const double rp = rcp_nr(piv);
-    if (lane == j) {
-#pragma unroll
-      for (int c = j + 1; c < kTsNB; ++c) cb[c] = g[c];
-    }
-    wave_lds_sync();
-    const double l = g[j] * rp;
-    if (i > j) {
-#pragma unroll
-      for (int c = j + 1; c < kTsNB; ++c) g[c] = fma(-l, cb[c], g[c]);
-      g[j] = l;
-    }
-    if (i == j) g[j] = piv;
-    wave_lds_sync();
-    if (lane == 0) { sg[j] = sgn; rdU[j] = rp; }
-  }
-  if (lane < kTsNB) {
-#pragma unroll
-    for (int c = 0; c < kTsNB; ++c) {
-      Lf[i * kCqLd + c] = c < i ? g[c] : (c == i ? 1.0 : 0.0);
-      U[i * kCqLd + c] = c >= i ? g[c] : 0.0;
-    }
-  }
 }
 
 // Inverse of a 32 x 32 triangular matrix in LDS (identity outside its
@@ -794,41 +839,52 @@ __device__ __forceinline__ void blk_tri_inverse(const double* L, const double* r
   }
 }
 
-// Y[row] = X[row] Li^T for nrows rows (Y = X R^-1 with R = L^T), four
-// threads per row, eight columns each; columns >= b are written as zero.
+typedef double qr_f64x4 __attribute__((ext_vector_type(4)));
+
+// One 16 x 16 tile of a product on the matrix cores: sum over k < K of
+// fa(m, k) fb(k, n) for the tile's rows m and columns n (0..15), by the
+// calling wave. The result's element v sits at row lk + 4 v, column lr
+// (lr = lane % 16, lk = lane / 16).
+template <int K, typename FA, typename FB>
+__device__ __forceinline__ qr_f64x4 qr_mfma_tile(FA fa, FB fb) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  qr_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < K; kk += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa(lr, kk + lk), fb(kk + lk, lr), acc, 0, 0, 0);
+  return acc;
+}
+
+// Y[r] = X[r] Li^T for nrows (a multiple of 16) rows in LDS, columns past b
+// zero; one 16 x 16 tile per wave at a time.
 __device__ __forceinline__ void cq_rows_transform(const double* X, const double* Li, double* Y, int nrows, int b) {
-  const int row = threadIdx.x >> 2, part = threadIdx.x & 3;
-  if (row >= nrows) return;
-  double y[8];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  for (int tile = wave; tile < (nrows / 16) * 2; tile += 4) {
+    const int r0 = (tile >> 1) * 16, c0 = (tile & 1) * 16;
+    const qr_f64x4 y = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return X[(r0 + m) * kCqLd + k]; },
+        [&](int k, int n) { return Li[(c0 + n) * kCqLd + k]; });
 #pragma unroll
-  for (int t = 0; t < 8; ++t) y[t] = 0.0;
-  for (int k = 0; k < b; ++k) {
-    const double xv = X[row * kCqLd + k];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) y[t] += xv * Li[(part + 4 * t) * kCqLd + k];
-  }
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int c = part + 4 * t;
-    Y[row * kCqLd + c] = c < b ? y[t] : 0.0;
+    for (int v = 0; v < 4; ++v) {
+      const int c = c0 + lr;
+      Y[(r0 + lk + 4 * v) * kCqLd + c] = c < b ? y[v] : 0.0;
+    }
   }
 }
 
 // Partial Gram Y^T Y (lower triangle) of the kCqRB rows in LDS (rows past
-// the panel are zero).
+// the panel are zero): waves 0..2 take the tiles (0,0), (1,0), (1,1).
 __device__ __forceinline__ void cq_gram_partial(const double* Y, double* __restrict__ Gout) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  if (wave > 2) return;
+  const int i0 = wave == 0 ? 0 : 16, c0 = wave == 2 ? 16 : 0;
+  const qr_f64x4 g = qr_mfma_tile<kCqRB>(
+      [&](int m, int k) { return Y[k * kCqLd + i0 + m]; },
+      [&](int k, int n) { return Y[k * kCqLd + c0 + n]; });
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int e = threadIdx.x + 256 * q;
-    const int i = e >> 5, c = e & 31;
-    if (c > i) continue;
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll 8
-    for (int r = 0; r < kCqRB; r += 2) {
-      s0 += Y[r * kCqLd + i] * Y[r * kCqLd + c];
-      s1 += Y[(r + 1) * kCqLd + i] * Y[(r + 1) * kCqLd + c];
-    }
-    Gout[(size_t)blockIdx.x * kCqG + e] = s0 + s1;
+  for (int v = 0; v < 4; ++v) {
+    const int i = i0 + lk + 4 * v, c = c0 + lr;
+    if (c <= i) Gout[(size_t)blockIdx.x * kCqG + i * kTsNB + c] = g[v];
   }
 }
 
@@ -858,7 +914,7 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   __shared__ double Lo[kTsNB * kCqLd];
   __shared__ double Li[kTsNB * kCqLd];
   __shared__ double rd[kTsNB];
-  __shared__ double cb[kTsNB];
+  __shared__ __attribute__((aligned(16))) double cb[4 * kTsNB];
   __shared__ int okf;
   const int tid = threadIdx.x;
   if (PASS > 0 && *flag) return;
@@ -927,29 +983,42 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
     X[i * kCqLd + c] = i < b && c < b ? q.Qb[(size_t)i * kTsNB + c] : 0.0;
   }
   __syncthreads();
-  for (int e = tid; e < kCqG; e += 256) {  // R[i][c] = (L1 L2)[c][i] -> Mb (upper)
-    const int i = e >> 5, c = e & 31;
-    double acc = 0.0;
-    if (i < b && c < b && i <= c)
-      for (int k = i; k <= c; ++k) acc += Ma[c * kCqLd + k] * Lo[k * kCqLd + i];
-    Mb[i * kCqLd + c] = acc;
+  {  // R[i][c] = (L1 L2)[c][i] -> Mb (upper): one 16 x 16 tile per wave
+    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int c0 = (wave >> 1) * 16, i0 = (wave & 1) * 16;
+    const qr_f64x4 r = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return Ma[(c0 + m) * kCqLd + k]; },
+        [&](int k, int nn) { return Lo[k * kCqLd + i0 + nn]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = c0 + lk + 4 * v, i = i0 + lr;
+      Mb[i * kCqLd + c] = i < b && c < b && i <= c ? r[v] : 0.0;
+    }
   }
   cq_rows_transform(X, Li, B, kTsNB, b);  // B = the top b x b of Q
   __syncthreads();
   cq_store(Mb, q.Rs);
   cq_store(Li, q.Li);
-  if (tid < 64) wv_lu(B, Lf, U, sg, rdU, cb);
+  blk_lu(B, Lf, U, sg, rdU, cb);
   __syncthreads();
   blk_tri_inverse<false>(Lf, nullptr, Ma, Y);  // Ma = L1f^-1
   blk_tri_inverse<true>(U, rdU, G, Y);         // G = U^-1
   cq_store(Lf, q.L1f);
   cq_store(G, q.Ui);
+  {  // T = -U S L1f^-T: one 16 x 16 tile per wave
+    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int i0 = (wave >> 1) * 16, j0 = (wave & 1) * 16;
+    const qr_f64x4 t = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return U[(i0 + m) * kCqLd + k] * sg[k]; },
+        [&](int k, int nn) { return Ma[(j0 + nn) * kCqLd + k]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + lk + 4 * v, j = j0 + lr;
+      q.T[i * kTsNB + j] = i < b && j < b && i <= j ? -t[v] : 0.0;
+    }
+  }
   for (int e = tid; e < kCqG; e += 256) {
     const int i = e >> 5, j = e & 31;
-    double t = 0.0;
-    if (i < b && j < b && i <= j)
-      for (int k = i; k <= j; ++k) t += U[i * kCqLd + k] * sg[k] * Ma[j * kCqLd + k];
-    q.T[e] = -t;
     q.Rbuf[e] = i < b && j < b && j >= i ? sg[i] * Mb[i * kCqLd + j] : 0.0;
   }
   if (tid == 0) *q.cnt = 0;
@@ -957,16 +1026,17 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
 
 // Rows of Q = Q1 R2^-1, the check of P - Q R, and the rows of V, Y1 = V T^T
 // and Y2 = V T (V's top b rows are L1 of the LU), kCqRows rows per
-// workgroup with 16 threads per row (columns part and part + 16). A
-// workgroup whose rows fail the check sets the flag.
+// workgroup. A workgroup whose rows fail the check sets the flag; with
+// put_r the workgroups of the top rows then write the accepted R into A
+// (the TSQR root does that when its kernels follow).
 constexpr int kCqRows = 16;
 
-__global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__ A, int n, int jb,
+__global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, int n, int jb,
                                                      int mp, int b, CqBuf q,
                                                      double* __restrict__ Vall,
                                                      double* __restrict__ Y1,
                                                      double* __restrict__ Y2,
-                                                     int* __restrict__ flag) {
+                                                     int* __restrict__ flag, int put_r) {
   __shared__ double X[kCqRows * kCqLd];
   __shared__ double Qr[kCqRows * kCqLd];
   __shared__ double Li[kTsNB * kCqLd];
@@ -974,6 +1044,7 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
   __shared__ double T[kTsNB * kCqLd];
   __shared__ double Lf[kTsNB * kCqLd];
   __shared__ double Rs[kTsNB * kCqLd];
+  __shared__ double Vt[kCqRows * kCqLd];
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (*flag) return;
@@ -989,18 +1060,17 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
     X[i * kCqLd + c] = r0 + i < mp && c < b ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
   }
   __syncthreads();
-  const int row = tid >> 4, part = tid & 15;
-  const int pr = r0 + row;
-  const int c0 = part, c1 = part + 16;
-  {  // Q = X Li^T
-    double y0 = 0.0, y1 = 0.0;
-    for (int k = 0; k < b; ++k) {
-      const double xv = X[row * kCqLd + k];
-      y0 = fma(xv, Li[c0 * kCqLd + k], y0);
-      y1 = fma(xv, Li[c1 * kCqLd + k], y1);
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  if (wave < 2) {  // Q = X Li^T, one 16-column tile per wave
+    const int c0 = wave * 16;
+    const qr_f64x4 y = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return X[m * kCqLd + k]; },
+        [&](int k, int nn) { return Li[(c0 + nn) * kCqLd + k]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = c0 + lr;
+      Qr[(lk + 4 * v) * kCqLd + c] = c < b ? y[v] : 0.0;
     }
-    Qr[row * kCqLd + c0] = c0 < b ? y0 : 0.0;
-    Qr[row * kCqLd + c1] = c1 < b ? y1 : 0.0;
   }
   __syncthreads();
   for (int e = tid; e < kCqRows * kTsNB; e += 256) {
@@ -1008,54 +1078,63 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(const double* __restrict__
     X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
   }
   __syncthreads();
-  {  // |P - Q R| against 2e-14 ||P||_F
+  if (wave < 2) {  // |P - Q R| against 2e-14 ||P||_F
+    const int c0 = wave * 16;
     const double thr = 2e-14 * *q.pnorm;
+    const qr_f64x4 d = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return Qr[m * kCqLd + k]; },
+        [&](int k, int nn) { return Rs[k * kCqLd + c0 + nn]; });
     bool fail = false;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int c = t ? c1 : c0;
-      if (c >= b) continue;
-      double acc = X[row * kCqLd + c];
-      for (int k = 0; k <= c; ++k) acc = fma(-Qr[row * kCqLd + k], Rs[k * kCqLd + c], acc);
-      if (!(fabs(acc) <= thr)) fail = true;
+    for (int v = 0; v < 4; ++v) {
+      const int r = lk + 4 * v, c = c0 + lr;
+      if (c < b && r0 + r < mp && !(fabs(X[r * kCqLd + c] - d[v]) <= thr)) fail = true;
     }
     if (fail) bad = 1;
+  } else {  // V rows: L1f for the top b rows, Q Ui below
+    const int c0 = (wave - 2) * 16;
+    const qr_f64x4 w = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return Qr[m * kCqLd + k]; },
+        [&](int k, int nn) { return Ui[k * kCqLd + c0 + nn]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = lk + 4 * v, c = c0 + lr, pr = r0 + r;
+      double val = 0.0;
+      if (c < b) val = pr < b ? (c < pr ? Lf[pr * kCqLd + c] : (c == pr ? 1.0 : 0.0)) : w[v];
+      Vt[r * kCqLd + c] = val;
+    }
   }
   __syncthreads();
   if (bad) {
     if (tid == 0) *flag = 1;
     return;
   }
-  // V rows -> X
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int c = t ? c1 : c0;
-    double v = 0.0;
-    if (c < b) {
-      if (pr < b) {
-        v = c < pr ? Lf[pr * kCqLd + c] : (c == pr ? 1.0 : 0.0);
-      } else {
-        for (int k = 0; k <= c; ++k) v = fma(Qr[row * kCqLd + k], Ui[k * kCqLd + c], v);
-      }
+  if (put_r && r0 < b) {
+    // the accepted R (S R) into the panel's top rows; this workgroup has
+    // read those rows already and no other workgroup reads them
+    for (int e = tid; e < kCqRows * kTsNB; e += 256) {
+      const int i = e >> 5, c = e & 31, pr = r0 + i;
+      if (pr < b && c < b && c >= pr) A[(int64_t)(jb + pr) * n + jb + c] = q.Rbuf[pr * kTsNB + c];
     }
-    X[row * kCqLd + c] = v;
   }
-  __syncthreads();
-  if (pr < mp) {
-    const int64_t off = (int64_t)(jb + pr) * n + jb;
+  {  // Y1 = V T^T (waves 0, 1), Y2 = V T (waves 2, 3); V itself with Y1
+    const int c0 = (wave & 1) * 16;
+    const bool second = wave >= 2;
+    const qr_f64x4 y = qr_mfma_tile<kTsNB>(
+        [&](int m, int k) { return Vt[m * kCqLd + k]; },
+        [&](int k, int nn) { return second ? T[k * kCqLd + c0 + nn] : T[(c0 + nn) * kCqLd + k]; });
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int c = t ? c1 : c0;
-      if (c >= b) continue;
-      double y1 = 0.0, y2 = 0.0;
-      for (int k = 0; k < b; ++k) {
-        const double v = X[row * kCqLd + k];
-        y1 = fma(v, T[c * kCqLd + k], y1);
-        y2 = fma(v, T[k * kCqLd + c], y2);
+    for (int v = 0; v < 4; ++v) {
+      const int r = lk + 4 * v, c = c0 + lr, pr = r0 + r;
+      if (pr < mp && c < b) {
+        const int64_t off = (int64_t)(jb + pr) * n + jb + c;
+        if (second) {
+          Y2[off] = y[v];
+        } else {
+          Y1[off] = y[v];
+          Vall[off] = Vt[r * kCqLd + c];
+        }
       }
-      Vall[off + c] = X[row * kCqLd + c];
-      Y1[off + c] = y1;
-      Y2[off + c] = y2;
     }
   }
 }
@@ -1079,6 +1158,7 @@ static size_t ts_ws_bytes(int m, int n) {
   b += 7 * al((size_t)kCqG * 8);                         // Li, L1, R, S R, L1f, U^-1, T
   b += al(8) + al(sizeof(int));                          // ||P||_F, arrival ticket
   b += al((size_t)((n + kTsNB - 1) / kTsNB) * sizeof(int));    // per-panel fallback flags
+  b += al((size_t)m * n * 8);                            // copy of A for a refactorisation
   return b + 256;
 }
 
@@ -1114,9 +1194,11 @@ static int launch_panel(const QrCfg& c, double* A, int m, int n, int jb, int b, 
   return TNX_ERR_UNSUPPORTED;
 }
 
-// Blocked QR with TSQR panels: per panel three small kernels (leaf, root,
-// apply) and two GEMMs for the trailing update; Q formed backwards with two
-// GEMMs per panel.
+// Blocked QR with Cholesky-QR2 panels (three kernels per panel) and two
+// GEMMs per panel for the trailing update; a panel that fails the
+// Cholesky-QR checks sends the whole factorisation to the TSQR panel path
+// (leaf, root, apply after every panel). Q formed backwards with two GEMMs
+// per panel.
 static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) {
   char* p = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
   double* Vall = (double*)p; p += al((size_t)m * n * 8);
@@ -1137,63 +1219,86 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   for (double** sm : smalls) { *sm = (double*)p; p += al((size_t)kCqG * 8); }
   q.pnorm = (double*)p; p += al(8);
   q.cnt = (int*)p; p += al(sizeof(int));
-  int* flags = (int*)p;
+  const int np = (n + kTsNB - 1) / kTsNB;
+  int* flags = (int*)p; p += al((size_t)np * sizeof(int));
+  double* Asave = (double*)p;
   hipLaunchKernelGGL(cq_init_kernel, dim3(1), dim3(1), 0, s, q.cnt);
   TNX_LAUNCH_CHECK();
   const size_t part_bytes = al((size_t)32 * kTsNB * n * 8);
-  const int np = (n + kTsNB - 1) / kTsNB;
-  for (int pi = 0; pi < np; ++pi) {
-    const int jb = pi * kTsNB;
-    const int b = std::min(kTsNB, n - jb);
-    const int mp = m - jb;
-    TsPlan pl;
-    if (!ts_plan(mp, &pl)) {
-      set_error("qr: panel of %d rows has no TSQR plan", mp);
-      return TNX_ERR_UNSUPPORTED;
+  // One pass over the panels. deferred: Cholesky-QR panels only (the rows
+  // kernel writes each accepted R into A); a panel that fails its checks
+  // only sets its flag, and the caller refactors. Otherwise the TSQR kernels
+  // follow every panel and factor the panels that were flagged.
+  auto panels = [&](bool deferred) -> int {
+    for (int pi = 0; pi < np; ++pi) {
+      const int jb = pi * kTsNB;
+      const int b = std::min(kTsNB, n - jb);
+      const int mp = m - jb;
+      TsPlan pl;
+      if (!ts_plan(mp, &pl)) {
+        set_error("qr: panel of %d rows has no TSQR plan", mp);
+        return TNX_ERR_UNSUPPORTED;
+      }
+      const int nblk = (mp + kCqRB - 1) / kCqRB;
+      int* fl = flags + pi;
+      hipLaunchKernelGGL(cq_pass_kernel<0>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb,
+                         mp, b, q, nblk, fl);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(cq_pass_kernel<1>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb,
+                         mp, b, q, nblk, fl);
+      TNX_LAUNCH_CHECK();
+      hipLaunchKernelGGL(cq_rows_kernel, dim3((mp + kCqRows - 1) / kCqRows), dim3(256), 0, s, A, n,
+                         jb, mp, b, q, Vall, Y1, Y2, fl, deferred ? 1 : 0);
+      TNX_LAUNCH_CHECK();
+      if (!deferred) {
+        hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b,
+                           pl.h, pl.nleaf, Rst, Qst, fl);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tsqr_root_kernel, dim3(1), dim3(512), 0, s, Rst, pl.nleaf, b, Qst, A, n,
+                           jb, Vall, Tst, Qtop, Ust, fl, (const double*)q.Rbuf);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tsqr_apply_kernel, dim3(pl.nleaf), dim3(256), 0, s, Qst, Qtop, Ust, Tst,
+                           mp, b, pl.h, pl.nleaf, n, jb, Vall, Y1, Y2, fl);
+        TNX_LAUNCH_CHECK();
+      }
+      const int nt = n - jb - b;
+      if (nt <= 0) continue;
+      const double* Vp = Vall + (size_t)jb * n + jb;
+      const double* Y1p = Y1 + (size_t)jb * n + jb;
+      double* At = A + (size_t)jb * n + jb + b;
+      // W = V^T A_trail   (b x nt)
+      GemmDesc g{};
+      g.M = b; g.N = nt; g.K = mp; g.batch = 1;
+      g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+      g.B = At; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+      g.C = Wb; g.tc = DT::F64; g.ldc = nt; g.sC = 0;
+      g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+      g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+      TNX_TRY(gemm(g, s));
+      // A_trail -= (V T^T) W
+      GemmDesc g3{};
+      g3.M = mp; g3.N = nt; g3.K = b; g3.batch = 1;
+      g3.A = Y1p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+      g3.B = Wb; g3.tb = DT::F64; g3.ldb = nt; g3.sB = 0; g3.b_ncontig = true;
+      g3.C = At; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+      g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+      TNX_TRY(gemm(g3, s));
     }
-    // Cholesky-QR passes; the TSQR kernels run only for a panel they flag
-    const int nblk = (mp + kCqRB - 1) / kCqRB;
-    int* fl = flags + pi;
-    hipLaunchKernelGGL(cq_pass_kernel<0>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp,
-                       b, q, nblk, fl);
-    TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cq_pass_kernel<1>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb, mp,
-                       b, q, nblk, fl);
-    TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cq_rows_kernel, dim3((mp + kCqRows - 1) / kCqRows), dim3(256), 0, s,
-                       (const double*)A, n, jb, mp, b, q, Vall, Y1, Y2, fl);
-    TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b, pl.h,
-                       pl.nleaf, Rst, Qst, fl);
-    TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tsqr_root_kernel, dim3(1), dim3(512), 0, s, Rst, pl.nleaf, b, Qst, A, n, jb,
-                       Vall, Tst, Qtop, Ust, fl, (const double*)q.Rbuf);
-    TNX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tsqr_apply_kernel, dim3(pl.nleaf), dim3(256), 0, s, Qst, Qtop, Ust, Tst, mp,
-                       b, pl.h, pl.nleaf, n, jb, Vall, Y1, Y2, fl);
-    TNX_LAUNCH_CHECK();
-    const int nt = n - jb - b;
-    if (nt <= 0) continue;
-    const double* Vp = Vall + (size_t)jb * n + jb;
-    const double* Y1p = Y1 + (size_t)jb * n + jb;
-    double* At = A + (size_t)jb * n + jb + b;
-    // W = V^T A_trail   (b x nt)
-    GemmDesc g{};
-    g.M = b; g.N = nt; g.K = mp; g.batch = 1;
-    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
-    g.B = At; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
-    g.C = Wb; g.tc = DT::F64; g.ldc = nt; g.sC = 0;
-    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
-    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
-    TNX_TRY(gemm(g, s));
-    // A_trail -= (V T^T) W
-    GemmDesc g3{};
-    g3.M = mp; g3.N = nt; g3.K = b; g3.batch = 1;
-    g3.A = Y1p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
-    g3.B = Wb; g3.tb = DT::F64; g3.ldb = nt; g3.sB = 0; g3.b_ncontig = true;
-    g3.C = At; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
-    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
-    TNX_TRY(gemm(g3, s));
+    return TNX_OK;
+  };
+  // Cholesky-QR panels without the TSQR kernels (three launches per panel
+  // that return at once when the panel is accepted), then one look at the
+  // flags; if a panel failed, A is restored and factored again with the
+  // TSQR fallback after every panel.
+  const size_t abytes = (size_t)m * n * 8;
+  TNX_HIP(hipMemcpyAsync(Asave, A, abytes, hipMemcpyDeviceToDevice, s));
+  TNX_TRY(panels(true));
+  std::vector<int> hf(np);
+  TNX_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s));
+  TNX_HIP(hipStreamSynchronize(s));
+  if (std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
+    TNX_HIP(hipMemcpyAsync(A, Asave, abytes, hipMemcpyDeviceToDevice, s));
+    TNX_TRY(panels(false));
   }
   {
     const int64_t nn = (int64_t)n * n;
