@@ -19,7 +19,9 @@
  *    matching *_workspace_bytes query.
  *  - Every call is stream-ordered on `stream` (a hipStream_t, NULL = the
  *    default stream) and reentrant. Calls that return host scalars
- *    (tnx_eigsh_heff1, tnx_syev_lowest's info) synchronise that stream.
+ *    (tnx_eigsh_heff1, tnx_syev_lowest's info) synchronise that stream, and
+ *    so do tnx_qr and tnx_gauge_right/left (one read of the QR's panel
+ *    flags).
  *  - Status: 0 on success, negative on failure; tnx_last_error() returns a
  *    thread-local message. No C++ exception crosses this boundary.
  */
@@ -80,7 +82,8 @@ int tnx_gemm(int M, int N, int K, int batch, double alpha, const void* A, int a_
  * default; TNX_AUTOTUNE=1 in the environment also turns it on): the first
  * GEMM of a shape missing from the table then times every configuration on a
  * scratch output it allocates with hipMalloc, waits on the stream, and adds
- * the fastest to the table. Do not use it under graph capture or where the
+ * the fastest to the table. Mode 1 tunes shapes of at least 2^28
+ * multiply-adds; tnx_set_autotune(2) (TNX_AUTOTUNE=2) tunes every shape. Do not use it under graph capture or where the
  * caller cannot allow allocation. tnx_tune_clear empties the table.
  * No reference counterpart (quimb leaves this to BLAS). */
 int tnx_set_autotune(int on);

@@ -26,8 +26,18 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ P, int splits, i
   const int64_t MN = (int64_t)M * N;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < MN;
        e += (int64_t)gridDim.x * blockDim.x) {
+    // in split order; eight loads in flight at a time
+    const double* pe = P + (int64_t)bat * splits * MN + e;
     double acc = 0.0;
-    for (int z = 0; z < splits; ++z) acc += P[((int64_t)bat * splits + z) * MN + e];
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pe[(int64_t)(z + u) * MN];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; z < splits; ++z) acc += pe[(int64_t)z * MN];
     const int64_t m = e / N, n = e % N;
     const int64_t off = (int64_t)b1 * sC + (int64_t)b2 * sC2 + m * ldc + n;
     double val = alpha * acc;
@@ -156,7 +166,7 @@ static int tune_load_locked(const char* path);
 static void init_from_env() {
   std::call_once(g_env_once, [] {
     const char* e = getenv("TNX_AUTOTUNE");
-    if (e && e[0] == '1') g_autotune.store(1);
+    if (e && (e[0] == '1' || e[0] == '2')) g_autotune.store(e[0] - '0');
     const char* cache = getenv("TNX_TUNE_CACHE");
     if (cache && cache[0]) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -172,7 +182,7 @@ NoTuneScope::~NoTuneScope() { --g_no_tune_depth; }
 
 static bool autotune_on() {
   init_from_env();
-  return g_no_tune_depth == 0 && g_autotune.load() == 1;
+  return g_no_tune_depth == 0 && g_autotune.load() >= 1;
 }
 
 // Tile table file: a comment line, then one line per GEMM shape key,
@@ -232,7 +242,9 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
   if (ci < 0) {
     ci = heuristic_cfg<PREC>(d, nsplit);
     const int64_t work = (int64_t)d.M * d.N * d.K * d.batch;
-    if (autotune_on() && work >= ((int64_t)1 << 28) && !d.bidx) {
+    // mode 1 tunes shapes of at least 2^28 multiply-adds, mode 2 every shape
+    const int64_t min_work = g_autotune.load() >= 2 ? 0 : ((int64_t)1 << 28);
+    if (autotune_on() && work >= min_work && !d.bidx) {
       // Time every config once on a scratch output (the caller's C is not
       // touched), keep the fastest. Runs once per shape key.
       // Split-K launches only write the partial-sum scratch (which the real
@@ -300,7 +312,7 @@ int gemm(const GemmDesc& d, hipStream_t s) {
 
 void set_autotune(int on) {
   init_from_env();
-  g_autotune.store(on ? 1 : 0);
+  g_autotune.store(on <= 0 ? 0 : (on >= 2 ? 2 : 1));
 }
 
 void tune_clear() {

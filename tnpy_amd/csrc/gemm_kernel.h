@@ -431,8 +431,26 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
     return;
   }
   char* Cb = (char*)g.C + ((size_t)b1 * g.sC + (size_t)b2 * g.sC2) * (g.tc == 0 ? 8 : 4);
+  // One 16-row band of the wave's tile at a time: with beta != 0 all of the
+  // band's old C values are loaded first, then combined and stored (a load
+  // placed after the previous element's store would wait for it, since the
+  // compiler cannot tell the addresses apart).
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < TM; ++i) {
+    double cv[TN][4];
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wm0 + i * 16 + row_of(v);
+        const int n = n0 + wn0 + j * 16 + lr;
+        cv[j][v] = 0.0;
+        if (g.beta != 0.0 && m < g.M && n < g.N) {
+          const int64_t off = (int64_t)m * g.ldc + n;
+          cv[j][v] = g.tc == 0 ? reinterpret_cast<const double*>(Cb)[off]
+                               : (double)reinterpret_cast<const float*>(Cb)[off];
+        }
+      }
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
@@ -442,17 +460,12 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
         if (m < g.M && n < g.N) {
           const int64_t off = (int64_t)m * g.ldc + n;
           double val = g.alpha * (double)acc[i][j][v];
-          if (g.tc == 0) {
-            double* c = reinterpret_cast<double*>(Cb) + off;
-            if (g.beta != 0.0) val += g.beta * *c;
-            *c = val;
-          } else {
-            float* c = reinterpret_cast<float*>(Cb) + off;
-            if (g.beta != 0.0) val += g.beta * (double)*c;
-            *c = (float)val;
-          }
+          if (g.beta != 0.0) val += g.beta * cv[j][v];
+          if (g.tc == 0) reinterpret_cast<double*>(Cb)[off] = val;
+          else reinterpret_cast<float*>(Cb)[off] = (float)val;
         }
       }
+  }
 }
 
 // ---------------------------------------------------------------------------
