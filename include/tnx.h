@@ -177,6 +177,11 @@ typedef struct {
   int max_matvecs;    /* iteration limit (all precisions) */
   int precision;      /* TNX_F64 / TNX_F32 / TNX_BF16X3 for the Krylov matvecs */
   int keep;           /* Ritz vectors kept at a thick restart (0 = default) */
+  int steps_per_check;  /* batched solver only: Lanczos steps queued between
+                           host reads of the per-chain statuses, each chain's
+                           convergence decided on the device (0 or 1 = read
+                           after every step, with the next step queued
+                           first); ignored by the single-problem solvers */
 } tnx_eigsh_opts_t;
 
 typedef struct {

@@ -173,9 +173,14 @@ def test_syev_batched_n1(gpu):
     assert np.all(host(evec) == 1.0)
 
 
-@pytest.mark.parametrize("chil, chir, B", [(16, 16, 4), (32, 64, 6), (64, 64, 5)])
-def test_eigsh_batched(gpu, chil, chir, B):
-    print(f"eigsh_batched chil={chil} chir={chir} B={B}", flush=True)
+@pytest.mark.parametrize("steps_per_check", [1, 8])
+@pytest.mark.parametrize("chil, chir, B", [(2, 4, 3), (16, 16, 4), (32, 64, 6), (64, 64, 5)])
+def test_eigsh_batched(gpu, chil, chir, B, steps_per_check):
+    """Every chain's pair against the oracle matvec and the single-chain
+    solver (same start: same eigenvalue and matvec count), with statuses
+    read after every step and with convergence decided on the device over
+    runs of 8 steps; (2, 4) exhausts its 16-dimensional space."""
+    print(f"eigsh_batched chil={chil} chir={chir} B={B} steps_per_check={steps_per_check}", flush=True)
     from tnpy_amd import _lib
 
     rng = np.random.default_rng(chil + 5 * chir + B)
@@ -187,7 +192,7 @@ def test_eigsh_batched(gpu, chil, chir, B):
     v0 = rng.standard_normal((B, N))
     vec = torch.empty(B, N, dtype=torch.float64, device="cuda")
     hvec = torch.empty(B, N, dtype=torch.float64, device="cuda")
-    opts = _lib.EigshOpts(1e-10, 32, 4000, 0, 8)
+    opts = _lib.EigshOpts(1e-10, 32, 4000, 0, 8, steps_per_check)
     infos = (_lib.EigshInfo * B)()
     wp, wb = ws(lib().tnx_eigsh_heff1_batched_workspace_bytes(chil, D, D, d, chir, 32, B))
     check(lib().tnx_eigsh_heff1_batched(dev(Ls).data_ptr(), dev(Ws).data_ptr(), dev(Rs).data_ptr(),

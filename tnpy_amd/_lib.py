@@ -41,6 +41,7 @@ class EigshOpts(C.Structure):
         ("max_matvecs", C.c_int),
         ("precision", C.c_int),
         ("keep", C.c_int),
+        ("steps_per_check", C.c_int),
     ]
 
 

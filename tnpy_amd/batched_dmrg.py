@@ -119,6 +119,7 @@ def _solve(owner, ws, stream, L, W, R, v0, dims, B, chains, site, tol):
     opts.max_matvecs = int(owner._max_matvecs)
     opts.precision = _lib.TNX_F64
     opts.keep = int(owner._keep)
+    opts.steps_per_check = int(owner._steps_per_check)
     infos = (_lib.EigshInfo * B)()
     wp, wb = ws.get(lib.tnx_eigsh_heff1_batched_workspace_bytes(l, Dl, Dr, d, r, opts.max_basis, B))
     v0 = v0.reshape(B, N)
@@ -307,6 +308,10 @@ class BatchedFiniteDMRG:
             each drawn as ``MatrixProductState.random(n, chi, d, seed)``.
         exact_solver_dim, max_basis, keep: as for ``FiniteDMRG``.
         max_matvecs: Lanczos matvec limit per site.
+        steps_per_check: Lanczos steps the batched solver queues between
+            reads of the per-chain statuses; each chain's convergence test
+            runs on the device and a converged chain's result is frozen
+            until the read (1 = read after every step).
         grouped: sweep all length groups in lockstep with one launch per
             kernel per step for all chains whose current site has the same
             shape (default). Rightward sweeps align the groups at the left
@@ -332,6 +337,7 @@ class BatchedFiniteDMRG:
         max_basis: int = 64,
         keep: int = 20,
         max_matvecs: int = 5000,
+        steps_per_check: int = 8,
         grouped: bool = True,
         concurrent: bool = False,
         device: Optional[torch.device] = None,
@@ -355,6 +361,7 @@ class BatchedFiniteDMRG:
         self._max_basis = max_basis
         self._keep = keep
         self._max_matvecs = max_matvecs
+        self._steps_per_check = steps_per_check
         self._grouped = grouped
         self._concurrent = concurrent
         self._ws = _lib.Workspace(self.device)

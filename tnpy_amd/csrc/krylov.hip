@@ -662,15 +662,32 @@ struct TriFinish {
 // With a step to finish (f.h1), ad is f.alpha and the new row's alpha and
 // the coupling come from the sums directly (every lane computes them; lane
 // 0 stores them for the later users), never through memory in this kernel.
+// Device-decided convergence for a batched run that reads the statuses only
+// every few steps (ctol >= 0; negative switches it off): st[12] keeps the running ||T|| estimate (the
+// criterion's anorm); once est <= ctol * anorm, st[10] = 1 and st[11] = n,
+// and the problem's later calls return at once, leaving its Ritz pair
+// (st[0..2], s) and the basis rows up to n as they were at convergence.
+__device__ __forceinline__ void chunk_mark(double* st, int n, double ctol) {
+  if (ctol >= 0.0 && threadIdx.x == 0) {
+    const double an = fmax(st[12], st[2]);
+    st[12] = an;
+    if (st[1] <= ctol * an) {
+      st[10] = 1.0;
+      st[11] = (double)n;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ ad,
                                                             const double* __restrict__ bd, int n,
                                                             const double* __restrict__ bres,
                                                             double* __restrict__ s,
                                                             double* st, PMap pm, int hinted,
-                                                            TriFinish f) {
+                                                            TriFinish f, double ctol) {
   const int64_t o = pm.off();
   ad += o; bd += o; s += o; st += o;
   if (bres) bres += o;
+  if (ctol >= 0.0 && st[10] != 0.0) return;
   const bool fin = f.h1 != nullptr;
   double a_last = 0.0, b_new = 0.0;
   if (fin) {
@@ -692,6 +709,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   const double bres_v = fin ? b_new : (bres ? *bres : 0.0);
   if (n >= 2 && n <= 64) {
     tridiag_lowest_reg(ad, bd, n, has_bres, bres_v, fin, a_last, s, st, hinted);
+    chunk_mark(st, n, ctol);
     return;
   }
   __shared__ double a[kTriMax], b[kTriMax], b2[kTriMax];
@@ -714,6 +732,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
       st[1] = has_bres ? fabs(bres_v) : 0.0;
       st[2] = fabs(a[0]);
     }
+    chunk_mark(st, n, ctol);
     return;
   }
   // Gershgorin bounds (lanes over rows, then wave min/max).
@@ -809,6 +828,7 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __rest
   __syncthreads();
   const double sg = shr[0];
   for (int i = lane; i < n; i += 64) s[i] = sg * z[i];
+  chunk_mark(st, n, ctol);
 }
 
 // ---------------------------------------------------------------------------
@@ -947,7 +967,7 @@ int syev_lowest(const double* M, int n, double* evec, double* eval_host, void* w
   hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, n, Vst, tau, dg, eg, kOne);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, kOne, 0, TriFinish{});
+                     (const double*)nullptr, z, st, kOne, 0, TriFinish{}, -1.0);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, n, z, evec, kOne,
                      (int64_t)0);
@@ -1557,7 +1577,7 @@ static int gd_correction(EigCtx& c, const float* V32, int J32, int keep, int m, 
       hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(256), 0, s, A, J, Vst, tau, dg, eg, kOne);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, dg, eg, J,
-                         (const double*)nullptr, zt, stg, kOne, 0, TriFinish{});
+                         (const double*)nullptr, zt, stg, kOne, 0, TriFinish{}, -1.0);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(ormtr_kernel, dim3(1), dim3(256), 0, s, Vst, tau, J, zt, k.s, kOne,
                          (int64_t)0);
@@ -1784,7 +1804,7 @@ static int eigsh_run(EigCtx& c, const double* v0, double* vec, double* hvec,
         const int mv_now = c.matvecs, mv64_now = c.matvecs64;
         hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, s, k.alpha, k.beta, J,
                            k.beta + j, k.s, k.st, kOne, 1,
-                           TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
+                           TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T}, -1.0);
         TNX_LAUNCH_CHECK();
         // V[j+1] = w / beta[j]
         hipLaunchKernelGGL(normalize_kernel<VT>, dim3(nb), dim3(256), 0, s, k.w, k.h3 + J,
@@ -1946,13 +1966,52 @@ __global__ void gather_status_kernel(const double* __restrict__ st, int first, i
   if (t < nst) out[(int64_t)blockIdx.y * nst + t] = st[pm.off() + first + t];
 }
 
+// Device-decided convergence (see chunk_mark): clear the flags and start
+// the running ||T|| estimate at |theta| of the start vector (st[4]).
+__global__ void chunk_init_kernel(double* st, PMap pm) {
+  st += pm.off();
+  if (threadIdx.x == 0) {
+    st[10] = 0.0;
+    st[11] = 0.0;
+    st[12] = fabs(st[4]);
+  }
+}
+
+// Results of the problems that finished within a run of steps: with Jc
+// the problem's row count at convergence (st[11]; Jl for one that did not
+// converge), vec = y = V[0..Jc) s and hvec = theta y + beta[Jc-1] s[Jc-1]
+// V[Jc] (the beta term dropped for a problem stopped by an exhausted
+// space). vec/hvec of logical problem p at p * N.
+__global__ void chunk_finalize_kernel(const double* __restrict__ V, int64_t N,
+                                      const double* __restrict__ s, const double* __restrict__ st,
+                                      const double* __restrict__ beta, int Jl, int exhausted,
+                                      double* __restrict__ vec, double* __restrict__ hvec, PMap pm) {
+  const int64_t o = pm.off();
+  V += o; s += o; st += o; beta += o;
+  const int64_t po = (int64_t)pm.pid() * N;
+  __shared__ double cs[kMaxBasis + 1];
+  const bool done = st[10] != 0.0;
+  const int Jc = done ? (int)st[11] : Jl;
+  if ((int)threadIdx.x < Jc) cs[threadIdx.x] = s[threadIdx.x];
+  __syncthreads();
+  const double th = st[0];
+  const double c = (!done && exhausted) ? 0.0 : beta[Jc - 1] * s[Jc - 1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int t = 0; t < Jc; ++t) acc += cs[t] * V[(int64_t)t * N + i];
+    vec[po + i] = acc;
+    if (hvec) hvec[po + i] = th * acc + c * V[(int64_t)Jc * N + i];
+  }
+}
+
 static size_t eigsh_b_problem_bytes(size_t N, int m) { return align256(eigsh_ws_bytes_n(N, 0, m)); }
 
 size_t eigsh_b_ws_bytes(int chil, int Dl, int Dr, int d, int chir, int max_basis, int batch) {
   const int m = std::max(3, std::min(max_basis, kMaxBasis));
   const size_t N = (size_t)chil * d * chir;
   return align256(heff1_ws_bytes_b(chil, Dl, Dr, d, chir, batch)) +
-         2 * align256((size_t)batch * sizeof(int)) + align256((size_t)batch * 4 * 8) +
+         2 * align256((size_t)batch * sizeof(int)) + align256((size_t)batch * 16 * 8) +
          eigsh_b_problem_bytes(N, m) * (size_t)batch + 256;
 }
 
@@ -1984,7 +2043,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
   p += align256(mvb);
   int* idx = (int*)p; p += align256((size_t)batch * sizeof(int));
   int* fidx = (int*)p; p += align256((size_t)batch * sizeof(int));
-  double* stat = (double*)p; p += align256((size_t)batch * 4 * 8);
+  double* stat = (double*)p; p += align256((size_t)batch * 16 * 8);
   const size_t P = eigsh_b_problem_bytes((size_t)N, m);
   const int64_t sp = (int64_t)(P / 8);
   KWork k;
@@ -1992,7 +2051,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
 
   std::vector<int> act(batch), fin;
   for (int b = 0; b < batch; ++b) act[b] = b;
-  std::vector<double> anorm(batch, 0.0), hs((size_t)batch * 4);
+  std::vector<double> anorm(batch, 0.0), hs((size_t)batch * 16);
   for (int b = 0; b < batch; ++b) info[b] = tnx_eigsh_info_t{0.0, 0.0, 0.0, 0, 0, 0, 0};
   int matvecs = 0, restarts = 0;
   bool any_fail = false;
@@ -2093,6 +2152,108 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     TNX_TRY(cgs_pass(k.V, N, JJ, k.w, k.h2, false, k.partials, k.h3, N, s, pm, cnt));
     return TNX_OK;
   };
+  // Device-decided convergence: with steps_per_check K > 1 the host queues
+  // up to K lockstep steps (bounded by the next restart, an exhausted space
+  // and the matvec limit) without reading statuses. Each problem's
+  // tridiagonal solve tests the criterion itself and freezes the problem's
+  // Ritz pair when it is met (chunk_mark), so a finished problem keeps its
+  // result while the others step on. One status read per run of steps then
+  // retires the finished problems (their Ritz vectors from the basis rows up
+  // to their own convergence) and restarts the rest when the basis is full.
+  const int K = opts->steps_per_check > 1 ? std::min(opts->steps_per_check, kMaxBasis) : 1;
+  if (K > 1 && cnt > 0) {
+    hipLaunchKernelGGL(chunk_init_kernel, dim3(1, cnt), dim3(64), 0, s, k.st, pm);
+    TNX_LAUNCH_CHECK();
+    int jc = 1;
+    while (cnt > 0) {
+      const int nb = vec_blocks(N, cnt);
+      const int mv0 = matvecs;
+      int steps = std::min(K, mb - jc);
+      steps = (int)std::min<int64_t>(steps, N - jc);
+      steps = std::max(0, std::min(steps, max_mv - matvecs));
+      for (int t = 0; t < steps; ++t) {
+        const int jj = jc + t, JJ = jj + 1;
+        TNX_TRY(front(jj));
+        hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, JJ,
+                           k.beta + jj, k.s, k.st, pm, 1,
+                           TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T}, tol);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w,
+                           k.h3 + JJ, k.V + (int64_t)(jj + 1) * N, N, pm);
+        TNX_LAUNCH_CHECK();
+      }
+      const int Jl = jc + steps;  // basis rows after the last step
+      TNX_TRY(status(0, 13, cnt));
+      const bool exhausted = Jl >= N;
+      const bool limit = matvecs >= max_mv;
+      std::vector<int> rest;
+      fin.clear();
+      for (int i = 0; i < cnt; ++i) {
+        const int b = act[i];
+        const double* h = &hs[(size_t)13 * i];
+        const bool done = h[10] != 0.0;
+        anorm[b] = std::max(anorm[b], h[12]);
+        if (done || exhausted || limit) {
+          const int Jc = done ? (int)h[11] : Jl;
+          fin.push_back(b);
+          info[b].eval = h[0];
+          info[b].resnorm = h[1];
+          info[b].matvecs = info[b].matvecs_f64 = mv0 + (Jc - jc);
+          info[b].restarts = restarts;
+          info[b].converged = (done || exhausted) ? 1 : 0;
+          any_fail |= !(done || exhausted);
+        } else {
+          rest.push_back(b);
+        }
+      }
+      if (!fin.empty()) {
+        TNX_TRY(upload(fidx, fin));
+        const int fc = (int)fin.size();
+        hipLaunchKernelGGL(chunk_finalize_kernel, dim3(vec_blocks(N, fc), fc), dim3(256), 0, s,
+                           k.V, N, k.s, k.st, k.beta, Jl, exhausted ? 1 : 0, vec, hvec,
+                           PMap{fidx, sp});
+        TNX_LAUNCH_CHECK();
+        act.swap(rest);
+        cnt = (int)act.size();
+        if (cnt == 0) break;
+        TNX_TRY(upload(idx, act));
+      }
+      if (Jl >= mb) {
+        const int jl = Jl - 1;
+        hipLaunchKernelGGL(tridiag_all_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, Jl,
+                           k.theta, k.S, kMaxBasis, pm);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tr_restart_kernel, dim3(1, cnt), dim3(64), 0, s, k.theta, k.S, k.beta, jl,
+                           keep, Jl, k.Cm, k.alpha, k.beta, k.T,
+                           reinterpret_cast<int*>(k.st + 8), pm);
+        TNX_LAUNCH_CHECK();
+        GemmDesc g{};
+        g.M = keep; g.N = (int)N; g.K = Jl; g.batch = cnt; g.bidx = idx;
+        g.A = k.Cm; g.ta = DT::F64; g.lda = kMaxBasis; g.sA = sp; g.a_kcontig = false;
+        g.B = k.V; g.tb = DT::F64; g.ldb = N; g.sB = sp; g.b_ncontig = true;
+        g.C = k.Y; g.tc = DT::F64; g.ldc = N; g.sC = sp;
+        g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+        TNX_TRY(gemm(g, s));
+        hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks((int64_t)keep * N, cnt), cnt),
+                           dim3(256), 0, s, k.Y, k.V, (int64_t)keep * N, pm, sp, sp);
+        TNX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(copy_map_kernel, dim3(vec_blocks(N, cnt), cnt), dim3(256), 0, s,
+                           k.V + (int64_t)Jl * N, k.V + (int64_t)keep * N, N, pm, sp, sp);
+        TNX_LAUNCH_CHECK();
+        restarts++;
+        jc = keep;
+      } else {
+        jc = Jl;
+      }
+    }
+    TNX_HIP(hipStreamSynchronize(s));
+    for (int b = 0; b < batch; ++b) info[b].anorm = anorm[b];
+    if (any_fail) {
+      set_error("eigsh_batched: matvec limit %d reached before convergence", max_mv);
+      return TNX_ERR_NOCONV;
+    }
+    return TNX_OK;
+  }
   int j = 1;
   bool queued = false;  // step j's front already launched for the active set
   while (cnt > 0) {
@@ -2102,7 +2263,7 @@ int eigsh_heff1_b(const double* L, const double* W, const double* R, const doubl
     const int mv_now = matvecs;
     hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, cnt), dim3(64), 0, s, k.alpha, k.beta, J,
                        k.beta + j, k.s, k.st, pm, 1,
-                       TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T});
+                       TriFinish{k.h1, k.h2, k.h3, k.alpha, k.beta, k.T}, -1.0);
     TNX_LAUNCH_CHECK();
     hipLaunchKernelGGL(normalize_kernel<double>, dim3(nb, cnt), dim3(256), 0, s, k.w, k.h3 + J,
                        k.V + (int64_t)(j + 1) * N, N, pm);
@@ -2236,7 +2397,7 @@ int syev_lowest_b(const double* M, int n, int batch, double* evec, double* eval_
   hipLaunchKernelGGL(sytrd_kernel, dim3(1, batch), dim3(256), 0, s, A, n, Vst, tau, dg, eg, pm);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1, batch), dim3(64), 0, s, dg, eg, n,
-                     (const double*)nullptr, z, st, pm, 0, TriFinish{});
+                     (const double*)nullptr, z, st, pm, 0, TriFinish{}, -1.0);
   TNX_LAUNCH_CHECK();
   hipLaunchKernelGGL(ormtr_kernel, dim3(1, batch), dim3(256), 0, s, Vst, tau, n, z, evec, pm,
                      (int64_t)n);
