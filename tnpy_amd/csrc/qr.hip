@@ -855,6 +855,16 @@ __device__ __forceinline__ qr_f64x4 qr_mfma_tile(FA fa, FB fb) {
   return acc;
 }
 
+// The same with K (a multiple of 4) known only at run time.
+template <typename FA, typename FB>
+__device__ __forceinline__ qr_f64x4 qr_mfma_tile_rt(FA fa, FB fb, int K) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  qr_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int kk = 0; kk < K; kk += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa(lr, kk + lk), fb(kk + lk, lr), acc, 0, 0, 0);
+  return acc;
+}
+
 // Y[r] = X[r] Li^T for nrows (a multiple of 16) rows in LDS, columns past b
 // zero; one 16 x 16 tile per wave at a time.
 __device__ __forceinline__ void cq_rows_transform(const double* X, const double* Li, double* Y, int nrows, int b) {
@@ -1139,6 +1149,68 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
   }
 }
 
+// Compact-WY factor of four consecutive panels (kQrGroup = 128 columns):
+// H_0 H_1 H_2 H_3 = I - V T V^T with T upper triangular, its diagonal
+// blocks the panels' T_j and above them, column block by column block,
+// T[0:32j, j] = -T[0:32j, 0:32j] (V_{0:j}^T V_j) T_j. G = V^T V (ld 128),
+// Tp the four panel factors (32 x 32 each, consecutive). One workgroup;
+// T and the product Z = (V_{0:j}^T V_j) T_j stay in LDS (rows padded by
+// one against bank conflicts), 16 x 16 MFMA tiles, one per wave at a time.
+constexpr int kQrGroup = 4 * kTsNB;
+constexpr int kQrGLd = kQrGroup + 1;
+__global__ __launch_bounds__(256) void qr_group_t_kernel(const double* __restrict__ G,
+                                                         const double* __restrict__ Tp,
+                                                         double* __restrict__ TG) {
+  __shared__ double Ts[kQrGroup * kQrGLd];
+  __shared__ double Z[3 * kTsNB * kCqLd];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  {
+    // the four diagonal blocks: 16 values per thread, loads issued together
+    double tv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tv[u] = Tp[tid + 256 * u];
+    for (int e = tid; e < kQrGroup * kQrGLd; e += 256) Ts[e] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + 256 * u, blk = e / kCqG, i = (e % kCqG) / kTsNB, c = e % kTsNB;
+      Ts[(blk * kTsNB + i) * kQrGLd + blk * kTsNB + c] = tv[u];
+    }
+  }
+  __syncthreads();
+  for (int j = 1; j < 4; ++j) {
+    const int rows = kTsNB * j, cj = kTsNB * j;
+    // Z = G[0:rows, cj:cj+32] T_j
+    for (int tile = wave; tile < (rows / 16) * 2; tile += 4) {
+      const int r0 = (tile >> 1) * 16, c0 = (tile & 1) * 16;
+      const qr_f64x4 z = qr_mfma_tile<kTsNB>(
+          [&](int m, int k) { return G[(r0 + m) * kQrGroup + cj + k]; },
+          [&](int k, int nn) { return Ts[(cj + k) * kQrGLd + cj + c0 + nn]; });
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Z[(r0 + lk + 4 * v) * kCqLd + c0 + lr] = z[v];
+    }
+    __syncthreads();
+    // T[0:rows, cj:cj+32] = -T[0:rows, 0:rows] Z
+    for (int tile = wave; tile < (rows / 16) * 2; tile += 4) {
+      const int r0 = (tile >> 1) * 16, c0 = (tile & 1) * 16;
+      const qr_f64x4 t = qr_mfma_tile_rt(
+          [&](int m, int k) { return Ts[(r0 + m) * kQrGLd + k]; },
+          [&](int k, int nn) { return Z[k * kCqLd + c0 + nn]; }, rows);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ts[(r0 + lk + 4 * v) * kQrGLd + cj + c0 + lr] = -t[v];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < kQrGroup * kQrGroup; e += 256)
+    TG[e] = Ts[(e / kQrGroup) * kQrGLd + e % kQrGroup];
+}
+
+__global__ void fill_zero_kernel(double* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 static bool ts_fits(int m, int n) {
   TsPlan p;
   return m >= n && ts_plan(m, &p);  // the first panel has the most rows
@@ -1159,6 +1231,9 @@ static size_t ts_ws_bytes(int m, int n) {
   b += al(8) + al(sizeof(int));                          // ||P||_F, arrival ticket
   b += al((size_t)((n + kTsNB - 1) / kTsNB) * sizeof(int));    // per-panel fallback flags
   b += al((size_t)m * n * 8);                            // copy of A for a refactorisation
+  b += al((size_t)((n + kTsNB - 1) / kTsNB) * kCqG * 8);  // every panel's T
+  b += 2 * al((size_t)kQrGroup * kQrGroup * 8);          // group T, V^T V
+  b += 2 * al((size_t)kQrGroup * n * 8);                 // group W, T W
   return b + 256;
 }
 
@@ -1221,7 +1296,22 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   q.cnt = (int*)p; p += al(sizeof(int));
   const int np = (n + kTsNB - 1) / kTsNB;
   int* flags = (int*)p; p += al((size_t)np * sizeof(int));
-  double* Asave = (double*)p;
+  double* Asave = (double*)p; p += al((size_t)m * n * 8);
+  double* Tall = (double*)p; p += al((size_t)np * kCqG * 8);
+  double* TG = (double*)p; p += al((size_t)kQrGroup * kQrGroup * 8);
+  double* GG = (double*)p; p += al((size_t)kQrGroup * kQrGroup * 8);
+  double* Wg = (double*)p; p += al((size_t)kQrGroup * n * 8);
+  double* W2g = (double*)p;
+  // Q is formed four panels at a time (one 128-wide block reflector) when
+  // n is a whole number of such groups and no panel fell back to TSQR; the
+  // group's V then needs zeros above each panel's diagonal block.
+  const bool group_q = n % kQrGroup == 0;
+  if (group_q) {
+    const int64_t mn = (int64_t)m * n;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((int)std::min<int64_t>((mn + 255) / 256, 4096)),
+                       dim3(256), 0, s, Vall, mn);
+    TNX_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(cq_init_kernel, dim3(1), dim3(1), 0, s, q.cnt);
   TNX_LAUNCH_CHECK();
   const size_t part_bytes = al((size_t)32 * kTsNB * n * 8);
@@ -1241,14 +1331,16 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
       }
       const int nblk = (mp + kCqRB - 1) / kCqRB;
       int* fl = flags + pi;
+      CqBuf qp = q;
+      qp.T = Tall + (size_t)pi * kCqG;  // kept for the grouped Q formation
       hipLaunchKernelGGL(cq_pass_kernel<0>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb,
-                         mp, b, q, nblk, fl);
+                         mp, b, qp, nblk, fl);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(cq_pass_kernel<1>, dim3(nblk), dim3(256), 0, s, (const double*)A, n, jb,
-                         mp, b, q, nblk, fl);
+                         mp, b, qp, nblk, fl);
       TNX_LAUNCH_CHECK();
       hipLaunchKernelGGL(cq_rows_kernel, dim3((mp + kCqRows - 1) / kCqRows), dim3(256), 0, s, A, n,
-                         jb, mp, b, q, Vall, Y1, Y2, fl, deferred ? 1 : 0);
+                         jb, mp, b, qp, Vall, Y1, Y2, fl, deferred ? 1 : 0);
       TNX_LAUNCH_CHECK();
       if (!deferred) {
         hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(pl.nleaf), dim3(256), 0, s, A, n, jb, mp, b,
@@ -1296,9 +1388,11 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   std::vector<int> hf(np);
   TNX_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s));
   TNX_HIP(hipStreamSynchronize(s));
+  bool refactored = false;
   if (std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
     TNX_HIP(hipMemcpyAsync(A, Asave, abytes, hipMemcpyDeviceToDevice, s));
     TNX_TRY(panels(false));
+    refactored = true;
   }
   {
     const int64_t nn = (int64_t)n * n;
@@ -1309,6 +1403,51 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
     const int b2 = (int)std::min<int64_t>((mn + 255) / 256, 4096);
     hipLaunchKernelGGL(eye_kernel, dim3(b2), dim3(256), 0, s, A, m, n);
     TNX_LAUNCH_CHECK();
+  }
+  if (group_q && !refactored) {
+    // Q = H_G0 ... H_G(ng-1) [I; 0] backwards, one 128-wide group at a time:
+    // T_G from the four panel factors and V_G^T V_G, then
+    // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's first.
+    for (int gi = n / kQrGroup - 1; gi >= 0; --gi) {
+      const int g0 = gi * kQrGroup;
+      const int mp = m - g0, nq = n - g0;
+      const double* Vp = Vall + (size_t)g0 * n + g0;
+      double* Qs = A + (size_t)g0 * n + g0;
+      GemmDesc gg{};
+      gg.M = kQrGroup; gg.N = kQrGroup; gg.K = mp; gg.batch = 1;
+      gg.A = Vp; gg.ta = DT::F64; gg.lda = n; gg.sA = 0; gg.a_kcontig = false;
+      gg.B = Vp; gg.tb = DT::F64; gg.ldb = n; gg.sB = 0; gg.b_ncontig = true;
+      gg.C = GG; gg.tc = DT::F64; gg.ldc = kQrGroup; gg.sC = 0;
+      gg.alpha = 1.0; gg.beta = 0.0; gg.prec = Prec::F64;
+      gg.splitk_ws = part; gg.splitk_ws_bytes = part_bytes;
+      TNX_TRY(gemm(gg, s));
+      hipLaunchKernelGGL(qr_group_t_kernel, dim3(1), dim3(256), 0, s, (const double*)GG,
+                         (const double*)(Tall + (size_t)(g0 / kTsNB) * kCqG), TG);
+      TNX_LAUNCH_CHECK();
+      GemmDesc g{};
+      g.M = kQrGroup; g.N = nq; g.K = mp; g.batch = 1;
+      g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+      g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+      g.C = Wg; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
+      g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+      g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+      TNX_TRY(gemm(g, s));
+      GemmDesc g2{};
+      g2.M = kQrGroup; g2.N = nq; g2.K = kQrGroup; g2.batch = 1;
+      g2.A = TG; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
+      g2.B = Wg; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
+      g2.C = W2g; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
+      g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
+      TNX_TRY(gemm(g2, s));
+      GemmDesc g3{};
+      g3.M = mp; g3.N = nq; g3.K = kQrGroup; g3.batch = 1;
+      g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+      g3.B = W2g; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+      g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+      g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+      TNX_TRY(gemm(g3, s));
+    }
+    return qr_fix_signs(A, m, R, n, sg, s);
   }
   for (int pi = np - 1; pi >= 0; --pi) {
     const int jb = pi * kTsNB;
