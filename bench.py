@@ -656,13 +656,14 @@ def main():
         print(json.dumps({"truncation": truncation_leg(args.chi, args.truncation, dev)}), flush=True)
         return
     if args.kernels_only:
-        rates = kernel_rates(args.chi, 5, 2, ["fp64", args.mixed])
+        rates = kernel_rates(args.chi, 5, 2, ["fp64", "fp32", "bf16x3"])
         print(json.dumps({"chi": args.chi, "rates": rates, "copy": copy_reference()}), flush=True)
         return
 
     n, chi = args.n, args.chi
-    # 1. kernel rates at chi
-    rates = kernel_rates(chi, 5, 2, ["fp64", args.mixed])
+    # 1. kernel rates at chi (the third arithmetic too, for its roofline)
+    other = "bf16x3" if args.mixed == "fp32" else "fp32"
+    rates = kernel_rates(chi, 5, 2, ["fp64", args.mixed, other])
     for p in ("fp64", args.mixed):
         r = rates[p]
         log(f"{p}: matvec {r['matvec_ms']:.3f} ms ({r['matvec_tflops']:.1f} TF/s), "
@@ -837,6 +838,15 @@ def main():
             "peak": PEAK_TFLOPS[args.mixed],
             "unit": "TFLOP/s",
             "frac": r["gemm_f32_operands_tflops"] / PEAK_TFLOPS[args.mixed],
+            "traffic": None,
+        },
+        f"roofline_{other}": {
+            "bound": "mfma",
+            "kernel": f"gemm_kernel ({other}) H_eff stage 1, M={2 * chi} N={5 * chi} K={chi}",
+            "achieved": rates[other]["gemm_tflops"],
+            "peak": PEAK_TFLOPS[other],
+            "unit": "TFLOP/s",
+            "frac": rates[other]["gemm_tflops"] / PEAK_TFLOPS[other],
             "traffic": None,
         },
         "gauge_move": {"ms": rates["gauge"]["ms"], "tflops": rates["gauge"]["tflops"],

@@ -225,6 +225,40 @@ def test_eigsh_batched(gpu, chil, chir, B, steps_per_check):
     print(f"matvec counts per chain: {sorted(counts)}")
 
 
+@pytest.mark.parametrize("steps_per_check", [1, 8])
+def test_eigsh_batched_matvec_limit(gpu, steps_per_check):
+    """A matvec limit below what the chains need: TNX_ERR_NOCONV, every
+    chain flagged unconverged, and the returned pair still consistent (unit
+    vector, hvec = H vec, eval its Rayleigh quotient)."""
+    from tnpy_amd import _lib
+
+    rng = np.random.default_rng(77)
+    chil = chir = 16
+    B, D, d = 3, 5, 2
+    N = chil * d * chir
+    Ls = np.stack([sym_env(rng, chil, D) for _ in range(B)])
+    Rs = np.stack([sym_env(rng, chir, D) for _ in range(B)])
+    Ws = mpo_stack(B, D, d, rng, symmetric=True)
+    v0 = rng.standard_normal((B, N))
+    vec = torch.empty(B, N, dtype=torch.float64, device="cuda")
+    hvec = torch.empty(B, N, dtype=torch.float64, device="cuda")
+    opts = _lib.EigshOpts(1e-14, 32, 10, 0, 8, steps_per_check)
+    infos = (_lib.EigshInfo * B)()
+    wp, wb = ws(lib().tnx_eigsh_heff1_batched_workspace_bytes(chil, D, D, d, chir, 32, B))
+    st = lib().tnx_eigsh_heff1_batched(dev(Ls).data_ptr(), dev(Ws).data_ptr(), dev(Rs).data_ptr(),
+                                       dev(v0).data_ptr(), vec.data_ptr(), hvec.data_ptr(), chil, D, D, d,
+                                       chir, B, C.byref(opts), infos, wp, wb, stream())
+    assert st == -3
+    vb, hb = host(vec), host(hvec)
+    for b in range(B):
+        info = infos[b]
+        assert info.converged == 0 and info.matvecs == 10
+        hy = o.heff_matvec(Ls[b], Ws[b], Rs[b], vb[b])
+        assert abs(np.linalg.norm(vb[b]) - 1.0) < 1e-12
+        assert np.linalg.norm(hy - hb[b]) <= 1e-11 * np.linalg.norm(hy)
+        assert abs(vb[b] @ hy - info.eval) <= 1e-11 * info.anorm
+
+
 def test_eigsh_batched_rejects_low_precision(gpu):
     from tnpy_amd import _lib
 
