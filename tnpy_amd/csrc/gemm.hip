@@ -258,26 +258,30 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
       if (g.partial || hipMalloc(&scratch, cbytes) == hipSuccess) {
         if (scratch) gt.C = scratch;
         gt.beta = 0.0;
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        float best = 1e30f;
-        for (int i = 0; i < Cfgs<PREC>::n; ++i) {
-          launch_index<PREC>(i, d, gt, nsplit, typed, s);  // warm
-          hipEventRecord(e0, s);
-          launch_index<PREC>(i, d, gt, nsplit, typed, s);
-          launch_index<PREC>(i, d, gt, nsplit, typed, s);
-          hipEventRecord(e1, s);
-          hipEventSynchronize(e1);
-          float ms = 0.f;
-          hipEventElapsedTime(&ms, e0, e1);
-          if (ms < best) { best = ms; ci = i; }
+        // A failure anywhere in the timing keeps the heuristic's choice.
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+          float best = 1e30f;
+          int pick = ci;
+          bool ok = true;
+          for (int i = 0; ok && i < Cfgs<PREC>::n; ++i) {
+            launch_index<PREC>(i, d, gt, nsplit, typed, s);  // warm
+            float ms = 0.f;
+            ok = hipEventRecord(e0, s) == hipSuccess;
+            launch_index<PREC>(i, d, gt, nsplit, typed, s);
+            launch_index<PREC>(i, d, gt, nsplit, typed, s);
+            ok = ok && hipEventRecord(e1, s) == hipSuccess &&
+                 hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (ok && ms < best) { best = ms; pick = i; }
+          }
+          if (ok) ci = pick;
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        hipStreamSynchronize(s);
-        if (scratch) hipFree(scratch);
-        hipGetLastError();
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(s);
+        if (scratch) (void)hipFree(scratch);
+        (void)hipGetLastError();
       }
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);

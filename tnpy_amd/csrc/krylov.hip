@@ -1362,7 +1362,7 @@ class PinnedLease {
     const size_t want = std::max<size_t>(n, 64);
     if (hipHostMalloc((void**)&p_, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
       p_ = nullptr;
-      hipGetLastError();
+      (void)hipGetLastError();
       return;
     }
     cap_ = want;
@@ -1396,11 +1396,11 @@ struct StepEvent {
   StepEvent() {
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
       e = nullptr;
-      hipGetLastError();
+      (void)hipGetLastError();
     }
   }
   ~StepEvent() {
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   }
   StepEvent(const StepEvent&) = delete;
   StepEvent& operator=(const StepEvent&) = delete;
