@@ -219,6 +219,7 @@ static int run_prec(const GemmDesc& d, hipStream_t s) {
   g.C = d.C; g.ldc = d.ldc; g.sC = d.sC; g.tc = (int)d.tc;
   g.alpha = d.alpha; g.beta = d.beta;
   g.tiles_n = 1;
+  g.group_m = 0;
   g.nb2 = d.nb2 > 0 ? d.nb2 : 1;
   g.sA2 = d.sA2; g.sB2 = d.sB2; g.sC2 = d.sC2;
   g.bidx = d.bidx;

@@ -35,6 +35,7 @@
 
 #include "tnx_common.h"
 
+#include <cmath>
 #include <type_traits>
 
 namespace tnx {
@@ -53,6 +54,7 @@ struct GemmArgs {
   void* C; int64_t ldc, sC; int tc;
   double alpha, beta;
   int tiles_n;  // number of tiles along N
+  int group_m;  // tile rows per group of the XCD-aware tile order (0: row-major order)
   int kc;       // K extent of one split (== K when not split)
   double* partial;  // split-K partial sums [batch][split][M][N], or null
   int nb2;          // second batch level (1 = none)
@@ -117,7 +119,24 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
   const int wn0 = (wave % WGN) * WN;
 
   const int bat = g.bidx ? g.bidx[blockIdx.y] : (int)blockIdx.y;
-  const int tile = blockIdx.x;
+  int tile = blockIdx.x;
+  if (g.group_m > 0) {
+    // XCD-aware tile order. Consecutive workgroups are dispatched to the 8
+    // XCDs in turn, each XCD with its own L2: workgroup x takes place x / 8
+    // of a contiguous range of tile ids kept for XCD x % 8. Tile ids run
+    // through groups of group_m tile rows, column by column inside a group,
+    // so an XCD's range is a compact block of tiles that share A row panels
+    // and B column panels in its L2. A bijection on [0, gridDim.x) for any
+    // grid size; each tile is computed exactly as in row-major order.
+    const int T = gridDim.x, x = blockIdx.x & 7, per = T >> 3, rem = T & 7;
+    const int lin = x * per + min(x, rem) + (blockIdx.x >> 3);
+    const int gw = g.group_m * g.tiles_n;
+    const int grp = lin / gw;
+    const int fm = grp * g.group_m;
+    const int gs = min(T / g.tiles_n - fm, g.group_m);
+    const int w = lin - grp * gw;
+    tile = (fm + w % gs) * g.tiles_n + w / gs;
+  }
   const int split = blockIdx.z;
   const int kbeg = split * g.kc;
   const int kend = min(g.K, kbeg + g.kc);
@@ -505,11 +524,27 @@ void gemm_launch_bf16(int ci, const GemmDesc& d, const GemmArgs& g, int splits, 
 // number of vectors, and one batch entry of each operand spans < 2 GiB.
 bool gemm_typed_ok(const GemmDesc& d, int kc);
 
+// Tile rows per group of the XCD-aware order: an XCD's ceil(tiles / 8)
+// tiles as close to the block a x b that minimises the panels it reads
+// (a BM + b BN with a b fixed: a = sqrt(tiles/8 * BN / BM)). Row-major
+// order (0) below 64 tiles, and where the batch or split index makes
+// blockIdx.x % 8 differ from the XCD (a grid.x not a multiple of 8 with
+// more than one batch entry or split).
+inline int gemm_group_m(int tiles_m, int tiles_n, int bm, int bn, int yz) {
+  const int tiles = tiles_m * tiles_n;
+  if (tiles < 64 || (yz > 1 && tiles % 8 != 0)) return 0;
+  const double per = (double)((tiles + 7) / 8);
+  const int a = (int)(std::sqrt(per * bn / bm) + 0.5);
+  return std::max(1, std::min(a, tiles_m));
+}
+
 template <int PREC, int BM, int BN, int WGM, int WGN, int BK, typename TA, typename TB>
 static void gemm_launch_cfg(const GemmDesc& d, const GemmArgs& g0, int splits, hipStream_t s) {
   GemmArgs g = g0;
   g.tiles_n = ceil_div(d.N, BN);
-  const int tiles = ceil_div(d.M, BM) * g.tiles_n;
+  const int tiles_m = ceil_div(d.M, BM);
+  const int tiles = tiles_m * g.tiles_n;
+  g.group_m = gemm_group_m(tiles_m, g.tiles_n, BM, BN, d.batch * splits);
   dim3 grid(tiles, d.batch, splits);
   dim3 block(64 * WGM * WGN);
   if (d.a_kcontig && d.b_ncontig)
