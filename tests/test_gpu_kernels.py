@@ -119,6 +119,25 @@ def test_gemm_typed_matches_generic(gpu, prec, ak, bn):
     assert np.array_equal(typed, generic)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("bt, M, N, K", [(1, 1030, 1950, 40), (1, 2000, 1000, 24), (3, 512, 512, 36),
+                                         (3, 520, 1000, 20), (1, 512, 512, 4096)])
+def test_gemm_large_grids(gpu, prec, bt, M, N, K):
+    """Grids of 64 or more output tiles run in the XCD-aware tile order:
+    tile counts that are and are not multiples of 8, batches and a K split.
+    C starts as non-zero data with beta = -0.5, so a tile left unwritten or
+    written from the wrong place fails the comparison."""
+    print(f"gemm large grid {prec} bt={bt} {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(bt * 1000 + M + N + K)
+    A = rng.standard_normal((bt, M, K))
+    B = rng.standard_normal((bt, K, N))
+    Cin = rng.standard_normal((bt, M, N))
+    Cd = dev(Cin)
+    check(L().tnx_gemm_f64(M, N, K, bt, 1.0, dev(A).data_ptr(), K, M * K, 1, dev(B).data_ptr(), N, K * N, 1,
+                           -0.5, Cd.data_ptr(), N, M * N, PREC[prec], stream()), "gemm")
+    assert relerr(host(Cd), A @ B - 0.5 * Cin) < TOL[prec]
+
+
 def test_gemm_batched(gpu):
     rng = np.random.default_rng(5)
     bt, M, N, K = 5, 40, 24, 33
