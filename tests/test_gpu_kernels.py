@@ -341,6 +341,70 @@ def test_qr_rank_deficient(gpu, m, n, rank, zero_cols):
     assert np.abs(Q @ Rh - A).max() <= 1e-12 * max(1.0, np.abs(A).max())
 
 
+def graded(rng, m, n, span):
+    """m x n with singular values 10^0 .. 10^-span (log-spaced) and random
+    singular vectors, the spectrum shape of converged DMRG site tensors."""
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sv = np.logspace(0, -span, n)
+    return (U * sv) @ V.T, sv
+
+
+@pytest.mark.parametrize("m, n, span", [(2048, 1024, 14), (2048, 1024, 8), (1536, 768, 16), (2048, 1000, 12)])
+def test_qr_graded(gpu, m, n, span):
+    """Panels of a graded matrix carry condition numbers up to 10^span:
+    Q stays orthonormal, Q R reproduces A to rounding, and R keeps A's
+    singular values to ~eps ||A|| (the grouped Q formation runs for
+    n = 1024 and 768, the panel-by-panel one for n = 1000)."""
+    print(f"qr graded {m}x{n} span {span}", flush=True)
+    rng = np.random.default_rng(m + n + span)
+    A, sv = graded(rng, m, n, span)
+    Ad = dev(A)
+    R = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    _, wp, wb = ws(L().tnx_qr_workspace_bytes(m, n))
+    check(L().tnx_qr(Ad.data_ptr(), m, n, R.data_ptr(), wp, wb, stream()), "qr")
+    Q = host(Ad)
+    Rh = host(R)
+    assert np.abs(np.tril(Rh, -1)).max() == 0.0
+    assert np.all(np.diag(Rh) >= 0.0)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+    assert relerr(Q @ Rh, A) < 1e-14
+    assert np.abs(np.linalg.svd(Rh, compute_uv=False) - sv).max() < 1e-13
+
+
+@pytest.mark.parametrize("right", [True, False])
+def test_gauge_moves_chi1024(gpu, right):
+    """Both gauge moves on a 1024 x 2 x 1024 site tensor with a graded
+    spectrum: the moved site is an isometry and the two-site product is
+    unchanged."""
+    print(f"gauge {'right' if right else 'left'} chi=1024", flush=True)
+    chi, d = 1024, 2
+    rng = np.random.default_rng(7 if right else 8)
+    if right:
+        A = graded(rng, chi * d, chi, 12)[0].reshape(chi, d, chi)
+        nxt = rng.standard_normal((chi, d, 4))
+        gp, wp, wb = ws(L().tnx_gauge_workspace_bytes(chi, d, chi))
+        Ad, out = dev(A), torch.empty(chi, d, 4, dtype=torch.float64, device="cuda")
+        check(L().tnx_gauge_right(Ad.data_ptr(), dev(nxt).data_ptr(), out.data_ptr(), chi, d, chi, d * 4,
+                                  wp, wb, stream()), "gauge_right")
+        Q = host(Ad).reshape(chi * d, chi)
+        assert np.abs(Q.T @ Q - np.eye(chi)).max() < 1e-13
+        before = A.reshape(chi * d, chi) @ nxt.reshape(chi, -1)
+        after = Q @ host(out).reshape(chi, -1)
+    else:
+        A = graded(rng, chi * d, chi, 12)[0].T.reshape(chi, d, chi)
+        prv = rng.standard_normal((4, d, chi))
+        gp, wp, wb = ws(L().tnx_gauge_workspace_bytes(chi, d, chi))
+        Ad, out = dev(A), torch.empty(4, d, chi, dtype=torch.float64, device="cuda")
+        check(L().tnx_gauge_left(Ad.data_ptr(), dev(prv).data_ptr(), out.data_ptr(), chi, d, chi, 4 * d,
+                                 wp, wb, stream()), "gauge_left")
+        Q = host(Ad).reshape(chi, d * chi)
+        assert np.abs(Q @ Q.T - np.eye(chi)).max() < 1e-13
+        before = prv.reshape(-1, chi) @ A.reshape(chi, -1)
+        after = host(out).reshape(-1, chi) @ Q
+    assert relerr(after, before) < 1e-14
+
+
 @pytest.mark.parametrize("chil, d, chir", [(1, 2, 2), (4, 2, 8), (32, 2, 32), (16, 2, 32), (60, 2, 60), (100, 2, 50)])
 def test_gauge_moves(gpu, chil, d, chir):
     rng = np.random.default_rng(chil * chir)
