@@ -19,9 +19,10 @@
  *    matching *_workspace_bytes query.
  *  - Every call is stream-ordered on `stream` (a hipStream_t, NULL = the
  *    default stream) and reentrant. Calls that return host scalars
- *    (tnx_eigsh_heff1, tnx_syev_lowest's info) synchronise that stream, and
- *    so do tnx_qr and tnx_gauge_right/left (one read of the QR's panel
- *    flags).
+ *    (tnx_eigsh_heff1, tnx_syev_lowest's info) synchronise that stream;
+ *    tnx_qr and tnx_gauge_right/left wait for one read of the QR's panel
+ *    flags, with the formation of Q and R already queued behind it (they
+ *    may return before that work ends, like any stream-ordered call).
  *  - Status: 0 on success, negative on failure; tnx_last_error() returns a
  *    thread-local message. No C++ exception crosses this boundary.
  */

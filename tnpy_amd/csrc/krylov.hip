@@ -1335,77 +1335,6 @@ struct EigCtx {
   size_t env32_bytes = 0;  // workspace reserved for them
 };
 
-// Pinned host blocks for the per-step status reads of the Lanczos solvers: a
-// copy into one can be waited for with an event while later work stays
-// queued behind it. A solve leases a block for its duration and returns it;
-// blocks come from one process-wide pool and are never freed (they live
-// until the process ends), so no HIP call runs when a host thread or the
-// process exits, and the pool never holds more blocks than solves ever ran
-// at once. get() is null when the allocation fails (the solver then reads
-// its status the plain way).
-class PinnedLease {
- public:
-  explicit PinnedLease(size_t n) {
-    {
-      std::lock_guard<std::mutex> lk(mu());
-      auto& fl = free_list();
-      size_t best = fl.size();
-      for (size_t i = 0; i < fl.size(); ++i)
-        if (fl[i].second >= n && (best == fl.size() || fl[i].second < fl[best].second)) best = i;
-      if (best < fl.size()) {
-        p_ = fl[best].first;
-        cap_ = fl[best].second;
-        fl.erase(fl.begin() + (std::ptrdiff_t)best);
-        return;
-      }
-    }
-    const size_t want = std::max<size_t>(n, 64);
-    if (hipHostMalloc((void**)&p_, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-      p_ = nullptr;
-      (void)hipGetLastError();
-      return;
-    }
-    cap_ = want;
-  }
-  ~PinnedLease() {
-    if (!p_) return;
-    std::lock_guard<std::mutex> lk(mu());
-    free_list().emplace_back(p_, cap_);
-  }
-  PinnedLease(const PinnedLease&) = delete;
-  PinnedLease& operator=(const PinnedLease&) = delete;
-  double* get() const { return p_; }
-
- private:
-  static std::mutex& mu() {
-    static std::mutex m;
-    return m;
-  }
-  static std::vector<std::pair<double*, size_t>>& free_list() {
-    // never destroyed: returning a block during process exit stays valid
-    static auto* v = new std::vector<std::pair<double*, size_t>>();
-    return *v;
-  }
-  double* p_ = nullptr;
-  size_t cap_ = 0;
-};
-
-// An event for the duration of one solve.
-struct StepEvent {
-  hipEvent_t e = nullptr;
-  StepEvent() {
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-      e = nullptr;
-      (void)hipGetLastError();
-    }
-  }
-  ~StepEvent() {
-    if (e) (void)hipEventDestroy(e);
-  }
-  StepEvent(const StepEvent&) = delete;
-  StepEvent& operator=(const StepEvent&) = delete;
-};
-
 static int apply_h(EigCtx& c, const void* x, int x_f32, double* y, int prec) {
   c.matvecs++;
   if (prec == TNX_F64) c.matvecs64++;

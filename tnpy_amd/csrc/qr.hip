@@ -1378,102 +1378,128 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
     }
     return TNX_OK;
   };
+  // R, then Q formed backwards: 128-wide groups, or one panel at a time.
+  auto finish = [&](bool grouped) -> int {
+    {
+      const int64_t nn = (int64_t)n * n;
+      const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 4096);
+      hipLaunchKernelGGL(extract_r_kernel, dim3(blocks), dim3(256), 0, s, A, n, R);
+      TNX_LAUNCH_CHECK();
+      const int64_t mn = (int64_t)m * n;
+      const int b2 = (int)std::min<int64_t>((mn + 255) / 256, 4096);
+      hipLaunchKernelGGL(eye_kernel, dim3(b2), dim3(256), 0, s, A, m, n);
+      TNX_LAUNCH_CHECK();
+    }
+    if (grouped) {
+      // Q = H_G0 ... H_G(ng-1) [I; 0] backwards, one 128-wide group at a time:
+      // T_G from the four panel factors and V_G^T V_G, then
+      // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's first.
+      for (int gi = n / kQrGroup - 1; gi >= 0; --gi) {
+        const int g0 = gi * kQrGroup;
+        const int mp = m - g0, nq = n - g0;
+        const double* Vp = Vall + (size_t)g0 * n + g0;
+        double* Qs = A + (size_t)g0 * n + g0;
+        GemmDesc gg{};
+        gg.M = kQrGroup; gg.N = kQrGroup; gg.K = mp; gg.batch = 1;
+        gg.A = Vp; gg.ta = DT::F64; gg.lda = n; gg.sA = 0; gg.a_kcontig = false;
+        gg.B = Vp; gg.tb = DT::F64; gg.ldb = n; gg.sB = 0; gg.b_ncontig = true;
+        gg.C = GG; gg.tc = DT::F64; gg.ldc = kQrGroup; gg.sC = 0;
+        gg.alpha = 1.0; gg.beta = 0.0; gg.prec = Prec::F64;
+        gg.splitk_ws = part; gg.splitk_ws_bytes = part_bytes;
+        TNX_TRY(gemm(gg, s));
+        hipLaunchKernelGGL(qr_group_t_kernel, dim3(1), dim3(256), 0, s, (const double*)GG,
+                           (const double*)(Tall + (size_t)(g0 / kTsNB) * kCqG), TG);
+        TNX_LAUNCH_CHECK();
+        GemmDesc g{};
+        g.M = kQrGroup; g.N = nq; g.K = mp; g.batch = 1;
+        g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+        g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+        g.C = Wg; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
+        g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+        g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+        TNX_TRY(gemm(g, s));
+        GemmDesc g2{};
+        g2.M = kQrGroup; g2.N = nq; g2.K = kQrGroup; g2.batch = 1;
+        g2.A = TG; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
+        g2.B = Wg; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
+        g2.C = W2g; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
+        g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
+        TNX_TRY(gemm(g2, s));
+        GemmDesc g3{};
+        g3.M = mp; g3.N = nq; g3.K = kQrGroup; g3.batch = 1;
+        g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+        g3.B = W2g; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+        g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+        g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+        TNX_TRY(gemm(g3, s));
+      }
+    } else {
+      for (int pi = np - 1; pi >= 0; --pi) {
+        const int jb = pi * kTsNB;
+        const int b = std::min(kTsNB, n - jb);
+        const int mp = m - jb;
+        const int nq = n - jb;
+        const double* Vp = Vall + (size_t)jb * n + jb;
+        const double* Y2p = Y2 + (size_t)jb * n + jb;
+        double* Qs = A + (size_t)jb * n + jb;
+        GemmDesc g{};
+        g.M = b; g.N = nq; g.K = mp; g.batch = 1;
+        g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
+        g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
+        g.C = Wb; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
+        g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
+        g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
+        TNX_TRY(gemm(g, s));
+        GemmDesc g3{};
+        g3.M = mp; g3.N = nq; g3.K = b; g3.batch = 1;
+        g3.A = Y2p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+        g3.B = Wb; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+        g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
+        g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
+        TNX_TRY(gemm(g3, s));
+      }
+    }
+    return qr_fix_signs(A, m, R, n, sg, s);
+  };
   // Cholesky-QR panels without the TSQR kernels (three launches per panel
   // that return at once when the panel is accepted), then one look at the
-  // flags; if a panel failed, A is restored and factored again with the
-  // TSQR fallback after every panel.
+  // flags. The flags are copied into a pinned block and the host waits for
+  // that copy only, while R and Q, formed as if every panel was accepted,
+  // stay queued behind it; if a panel failed, A is restored, factored again
+  // with the TSQR fallback after every panel, and R and Q formed again. A
+  // thread whose last QR fell back waits for the flags before queueing R
+  // and Q (fallbacks come in runs: the same kind of matrix again).
+  static thread_local bool last_fell_back = false;
   const size_t abytes = (size_t)m * n * 8;
   TNX_HIP(hipMemcpyAsync(Asave, A, abytes, hipMemcpyDeviceToDevice, s));
   TNX_TRY(panels(true));
-  std::vector<int> hf(np);
-  TNX_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s));
-  TNX_HIP(hipStreamSynchronize(s));
-  bool refactored = false;
-  if (std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; })) {
+  PinnedLease pin(((size_t)np + 1) / 2);
+  StepEvent fev;
+  bool queued = false, failed = false;
+  if (pin.get() && fev.e && !last_fell_back) {
+    int* hf = reinterpret_cast<int*>(pin.get());
+    TNX_HIP(hipMemcpyAsync(hf, flags, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipEventRecord(fev.e, s));
+    const int st = finish(group_q);
+    // the copy must land before the block goes back to the pool
+    const hipError_t we = hipEventSynchronize(fev.e);
+    TNX_TRY(st);
+    TNX_HIP(we);
+    queued = true;
+    failed = std::any_of(hf, hf + np, [](int f) { return f != 0; });
+  } else {
+    std::vector<int> hf(np);
+    TNX_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s));
+    TNX_HIP(hipStreamSynchronize(s));
+    failed = std::any_of(hf.begin(), hf.end(), [](int f) { return f != 0; });
+  }
+  last_fell_back = failed;
+  if (failed) {
     TNX_HIP(hipMemcpyAsync(A, Asave, abytes, hipMemcpyDeviceToDevice, s));
     TNX_TRY(panels(false));
-    refactored = true;
+    return finish(false);
   }
-  {
-    const int64_t nn = (int64_t)n * n;
-    const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 4096);
-    hipLaunchKernelGGL(extract_r_kernel, dim3(blocks), dim3(256), 0, s, A, n, R);
-    TNX_LAUNCH_CHECK();
-    const int64_t mn = (int64_t)m * n;
-    const int b2 = (int)std::min<int64_t>((mn + 255) / 256, 4096);
-    hipLaunchKernelGGL(eye_kernel, dim3(b2), dim3(256), 0, s, A, m, n);
-    TNX_LAUNCH_CHECK();
-  }
-  if (group_q && !refactored) {
-    // Q = H_G0 ... H_G(ng-1) [I; 0] backwards, one 128-wide group at a time:
-    // T_G from the four panel factors and V_G^T V_G, then
-    // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's first.
-    for (int gi = n / kQrGroup - 1; gi >= 0; --gi) {
-      const int g0 = gi * kQrGroup;
-      const int mp = m - g0, nq = n - g0;
-      const double* Vp = Vall + (size_t)g0 * n + g0;
-      double* Qs = A + (size_t)g0 * n + g0;
-      GemmDesc gg{};
-      gg.M = kQrGroup; gg.N = kQrGroup; gg.K = mp; gg.batch = 1;
-      gg.A = Vp; gg.ta = DT::F64; gg.lda = n; gg.sA = 0; gg.a_kcontig = false;
-      gg.B = Vp; gg.tb = DT::F64; gg.ldb = n; gg.sB = 0; gg.b_ncontig = true;
-      gg.C = GG; gg.tc = DT::F64; gg.ldc = kQrGroup; gg.sC = 0;
-      gg.alpha = 1.0; gg.beta = 0.0; gg.prec = Prec::F64;
-      gg.splitk_ws = part; gg.splitk_ws_bytes = part_bytes;
-      TNX_TRY(gemm(gg, s));
-      hipLaunchKernelGGL(qr_group_t_kernel, dim3(1), dim3(256), 0, s, (const double*)GG,
-                         (const double*)(Tall + (size_t)(g0 / kTsNB) * kCqG), TG);
-      TNX_LAUNCH_CHECK();
-      GemmDesc g{};
-      g.M = kQrGroup; g.N = nq; g.K = mp; g.batch = 1;
-      g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
-      g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
-      g.C = Wg; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
-      g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
-      g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
-      TNX_TRY(gemm(g, s));
-      GemmDesc g2{};
-      g2.M = kQrGroup; g2.N = nq; g2.K = kQrGroup; g2.batch = 1;
-      g2.A = TG; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
-      g2.B = Wg; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
-      g2.C = W2g; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
-      g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
-      TNX_TRY(gemm(g2, s));
-      GemmDesc g3{};
-      g3.M = mp; g3.N = nq; g3.K = kQrGroup; g3.batch = 1;
-      g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
-      g3.B = W2g; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
-      g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
-      g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
-      TNX_TRY(gemm(g3, s));
-    }
-    return qr_fix_signs(A, m, R, n, sg, s);
-  }
-  for (int pi = np - 1; pi >= 0; --pi) {
-    const int jb = pi * kTsNB;
-    const int b = std::min(kTsNB, n - jb);
-    const int mp = m - jb;
-    const int nq = n - jb;
-    const double* Vp = Vall + (size_t)jb * n + jb;
-    const double* Y2p = Y2 + (size_t)jb * n + jb;
-    double* Qs = A + (size_t)jb * n + jb;
-    GemmDesc g{};
-    g.M = b; g.N = nq; g.K = mp; g.batch = 1;
-    g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
-    g.B = Qs; g.tb = DT::F64; g.ldb = n; g.sB = 0; g.b_ncontig = true;
-    g.C = Wb; g.tc = DT::F64; g.ldc = nq; g.sC = 0;
-    g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
-    g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
-    TNX_TRY(gemm(g, s));
-    GemmDesc g3{};
-    g3.M = mp; g3.N = nq; g3.K = b; g3.batch = 1;
-    g3.A = Y2p; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
-    g3.B = Wb; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
-    g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
-    g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
-    TNX_TRY(gemm(g3, s));
-  }
-  return qr_fix_signs(A, m, R, n, sg, s);
+  return queued ? TNX_OK : finish(group_q);
 }
 
 int qr(double* A, int m, int n, double* R, void* ws, size_t ws_bytes, hipStream_t s) {
