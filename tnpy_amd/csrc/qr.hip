@@ -1161,6 +1161,10 @@ constexpr int kQrGLd = kQrGroup + 1;
 __global__ __launch_bounds__(256) void qr_group_t_kernel(const double* __restrict__ G,
                                                          const double* __restrict__ Tp,
                                                          double* __restrict__ TG) {
+  // one workgroup per group: its V^T V, its four panel T's, its T_G
+  G += (size_t)blockIdx.x * kQrGroup * kQrGroup;
+  Tp += (size_t)blockIdx.x * (kQrGroup / kTsNB) * kCqG;
+  TG += (size_t)blockIdx.x * kQrGroup * kQrGroup;
   __shared__ double Ts[kQrGroup * kQrGLd];
   __shared__ double Z[3 * kTsNB * kCqLd];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
@@ -1232,7 +1236,7 @@ static size_t ts_ws_bytes(int m, int n) {
   b += al((size_t)((n + kTsNB - 1) / kTsNB) * sizeof(int));    // per-panel fallback flags
   b += al((size_t)m * n * 8);                            // copy of A for a refactorisation
   b += al((size_t)((n + kTsNB - 1) / kTsNB) * kCqG * 8);  // every panel's T
-  b += 2 * al((size_t)kQrGroup * kQrGroup * 8);          // group T, V^T V
+  b += 2 * al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);  // every group's T, V^T V
   b += 2 * al((size_t)kQrGroup * n * 8);                 // group W, T W
   return b + 256;
 }
@@ -1298,8 +1302,8 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   int* flags = (int*)p; p += al((size_t)np * sizeof(int));
   double* Asave = (double*)p; p += al((size_t)m * n * 8);
   double* Tall = (double*)p; p += al((size_t)np * kCqG * 8);
-  double* TG = (double*)p; p += al((size_t)kQrGroup * kQrGroup * 8);
-  double* GG = (double*)p; p += al((size_t)kQrGroup * kQrGroup * 8);
+  double* TG = (double*)p; p += al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);
+  double* GG = (double*)p; p += al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);
   double* Wg = (double*)p; p += al((size_t)kQrGroup * n * 8);
   double* W2g = (double*)p;
   // Q is formed four panels at a time (one 128-wide block reflector) when
@@ -1392,24 +1396,31 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
     }
     if (grouped) {
       // Q = H_G0 ... H_G(ng-1) [I; 0] backwards, one 128-wide group at a time:
-      // T_G from the four panel factors and V_G^T V_G, then
-      // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's first.
-      for (int gi = n / kQrGroup - 1; gi >= 0; --gi) {
+      // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's
+      // first. Every group's V_G^T V_G first, in one batched GEMM over all m
+      // rows (V is zero above each panel's diagonal block, so the rows above
+      // the group add nothing), then every T_G (from its four panel T's and
+      // V_G^T V_G) in one launch of a workgroup per group.
+      const int ng = n / kQrGroup;
+      {
+        GemmDesc gg{};
+        gg.M = kQrGroup; gg.N = kQrGroup; gg.K = m; gg.batch = ng;
+        gg.A = Vall; gg.ta = DT::F64; gg.lda = n; gg.sA = kQrGroup; gg.a_kcontig = false;
+        gg.B = Vall; gg.tb = DT::F64; gg.ldb = n; gg.sB = kQrGroup; gg.b_ncontig = true;
+        gg.C = GG; gg.tc = DT::F64; gg.ldc = kQrGroup; gg.sC = (int64_t)kQrGroup * kQrGroup;
+        gg.alpha = 1.0; gg.beta = 0.0; gg.prec = Prec::F64;
+        gg.splitk_ws = part; gg.splitk_ws_bytes = part_bytes;
+        TNX_TRY(gemm(gg, s));
+        hipLaunchKernelGGL(qr_group_t_kernel, dim3(ng), dim3(256), 0, s, (const double*)GG,
+                           (const double*)Tall, TG);
+        TNX_LAUNCH_CHECK();
+      }
+      for (int gi = ng - 1; gi >= 0; --gi) {
         const int g0 = gi * kQrGroup;
         const int mp = m - g0, nq = n - g0;
         const double* Vp = Vall + (size_t)g0 * n + g0;
         double* Qs = A + (size_t)g0 * n + g0;
-        GemmDesc gg{};
-        gg.M = kQrGroup; gg.N = kQrGroup; gg.K = mp; gg.batch = 1;
-        gg.A = Vp; gg.ta = DT::F64; gg.lda = n; gg.sA = 0; gg.a_kcontig = false;
-        gg.B = Vp; gg.tb = DT::F64; gg.ldb = n; gg.sB = 0; gg.b_ncontig = true;
-        gg.C = GG; gg.tc = DT::F64; gg.ldc = kQrGroup; gg.sC = 0;
-        gg.alpha = 1.0; gg.beta = 0.0; gg.prec = Prec::F64;
-        gg.splitk_ws = part; gg.splitk_ws_bytes = part_bytes;
-        TNX_TRY(gemm(gg, s));
-        hipLaunchKernelGGL(qr_group_t_kernel, dim3(1), dim3(256), 0, s, (const double*)GG,
-                           (const double*)(Tall + (size_t)(g0 / kTsNB) * kCqG), TG);
-        TNX_LAUNCH_CHECK();
+        const double* TGi = TG + (size_t)gi * kQrGroup * kQrGroup;
         GemmDesc g{};
         g.M = kQrGroup; g.N = nq; g.K = mp; g.batch = 1;
         g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
@@ -1420,7 +1431,7 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
         TNX_TRY(gemm(g, s));
         GemmDesc g2{};
         g2.M = kQrGroup; g2.N = nq; g2.K = kQrGroup; g2.batch = 1;
-        g2.A = TG; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
+        g2.A = TGi; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
         g2.B = Wg; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
         g2.C = W2g; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
         g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
