@@ -1237,7 +1237,7 @@ static size_t ts_ws_bytes(int m, int n) {
   b += al((size_t)m * n * 8);                            // copy of A for a refactorisation
   b += al((size_t)((n + kTsNB - 1) / kTsNB) * kCqG * 8);  // every panel's T
   b += 2 * al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);  // every group's T, V^T V
-  b += 2 * al((size_t)kQrGroup * n * 8);                 // group W, T W
+  b += al((size_t)kQrGroup * n * 8);                     // group W
   return b + 256;
 }
 
@@ -1304,8 +1304,7 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
   double* Tall = (double*)p; p += al((size_t)np * kCqG * 8);
   double* TG = (double*)p; p += al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);
   double* GG = (double*)p; p += al((size_t)(n / kQrGroup + 1) * kQrGroup * kQrGroup * 8);
-  double* Wg = (double*)p; p += al((size_t)kQrGroup * n * 8);
-  double* W2g = (double*)p;
+  double* Wg = (double*)p;
   // Q is formed four panels at a time (one 128-wide block reflector) when
   // n is a whole number of such groups and no panel fell back to TSQR; the
   // group's V then needs zeros above each panel's diagonal block.
@@ -1396,11 +1395,13 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
     }
     if (grouped) {
       // Q = H_G0 ... H_G(ng-1) [I; 0] backwards, one 128-wide group at a time:
-      // Q_s -= V_G (T_G (V_G^T Q_s)) on rows and columns from the group's
-      // first. Every group's V_G^T V_G first, in one batched GEMM over all m
-      // rows (V is zero above each panel's diagonal block, so the rows above
-      // the group add nothing), then every T_G (from its four panel T's and
-      // V_G^T V_G) in one launch of a workgroup per group.
+      // Q_s -= (V_G T_G) (V_G^T Q_s) on rows and columns from the group's
+      // first. What depends on the panels only comes first, batched over the
+      // groups: every V_G^T V_G in one GEMM over all m rows (V is zero above
+      // each panel's diagonal block, so the rows above the group add
+      // nothing), every T_G (from its four panel T's and V_G^T V_G) in one
+      // launch of a workgroup per group, and every Y_G = V_G T_G in one GEMM
+      // (into Y2, which only the panel-by-panel form reads).
       const int ng = n / kQrGroup;
       {
         GemmDesc gg{};
@@ -1414,13 +1415,21 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
         hipLaunchKernelGGL(qr_group_t_kernel, dim3(ng), dim3(256), 0, s, (const double*)GG,
                            (const double*)Tall, TG);
         TNX_LAUNCH_CHECK();
+        GemmDesc gy{};
+        gy.M = m; gy.N = kQrGroup; gy.K = kQrGroup; gy.batch = ng;
+        gy.A = Vall; gy.ta = DT::F64; gy.lda = n; gy.sA = kQrGroup; gy.a_kcontig = true;
+        gy.B = TG; gy.tb = DT::F64; gy.ldb = kQrGroup; gy.sB = (int64_t)kQrGroup * kQrGroup;
+        gy.b_ncontig = true;
+        gy.C = Y2; gy.tc = DT::F64; gy.ldc = n; gy.sC = kQrGroup;
+        gy.alpha = 1.0; gy.beta = 0.0; gy.prec = Prec::F64;
+        TNX_TRY(gemm(gy, s));
       }
       for (int gi = ng - 1; gi >= 0; --gi) {
         const int g0 = gi * kQrGroup;
         const int mp = m - g0, nq = n - g0;
         const double* Vp = Vall + (size_t)g0 * n + g0;
         double* Qs = A + (size_t)g0 * n + g0;
-        const double* TGi = TG + (size_t)gi * kQrGroup * kQrGroup;
+        const double* Yp = Y2 + (size_t)g0 * n + g0;
         GemmDesc g{};
         g.M = kQrGroup; g.N = nq; g.K = mp; g.batch = 1;
         g.A = Vp; g.ta = DT::F64; g.lda = n; g.sA = 0; g.a_kcontig = false;
@@ -1429,17 +1438,10 @@ static int qr_tsqr(double* A, int m, int n, double* R, void* ws, hipStream_t s) 
         g.alpha = 1.0; g.beta = 0.0; g.prec = Prec::F64;
         g.splitk_ws = part; g.splitk_ws_bytes = part_bytes;
         TNX_TRY(gemm(g, s));
-        GemmDesc g2{};
-        g2.M = kQrGroup; g2.N = nq; g2.K = kQrGroup; g2.batch = 1;
-        g2.A = TGi; g2.ta = DT::F64; g2.lda = kQrGroup; g2.sA = 0; g2.a_kcontig = true;
-        g2.B = Wg; g2.tb = DT::F64; g2.ldb = nq; g2.sB = 0; g2.b_ncontig = true;
-        g2.C = W2g; g2.tc = DT::F64; g2.ldc = nq; g2.sC = 0;
-        g2.alpha = 1.0; g2.beta = 0.0; g2.prec = Prec::F64;
-        TNX_TRY(gemm(g2, s));
         GemmDesc g3{};
         g3.M = mp; g3.N = nq; g3.K = kQrGroup; g3.batch = 1;
-        g3.A = Vp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
-        g3.B = W2g; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
+        g3.A = Yp; g3.ta = DT::F64; g3.lda = n; g3.sA = 0; g3.a_kcontig = true;
+        g3.B = Wg; g3.tb = DT::F64; g3.ldb = nq; g3.sB = 0; g3.b_ncontig = true;
         g3.C = Qs; g3.tc = DT::F64; g3.ldc = n; g3.sC = 0;
         g3.alpha = -1.0; g3.beta = 1.0; g3.prec = Prec::F64;
         TNX_TRY(gemm(g3, s));
