@@ -373,12 +373,40 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int nk = (kend - kbeg + BK - 1) / BK;
+  const int lr = lane & 15;   // row (A) / column (B) inside a 16x16 tile
+  const int lk = lane >> 4;   // k group of the lane
+  //   f32 MFMAs: row = 4*lk + v ;  f64 MFMA: row = lk + 4*v ; column = lane%16
+  auto row_of = [&](int v) { return (PREC == 0) ? (lk + 4 * v) : (lk * 4 + v); };
+  char* Cb = (char*)g.C + ((size_t)b1 * g.sC + (size_t)b2 * g.sC2) * (g.tc == 0 ? 8 : 4);
+  auto load_c = [&](int i, int j, int v) -> double {
+    const int m = m0 + wm0 + i * 16 + row_of(v);
+    const int n = n0 + wn0 + j * 16 + lr;
+    if (!(m < g.M && n < g.N)) return 0.0;
+    const int64_t off = (int64_t)m * g.ldc + n;
+    return g.tc == 0 ? reinterpret_cast<const double*>(Cb)[off]
+                     : (double)reinterpret_cast<const float*>(Cb)[off];
+  };
+  // Small wave tiles (at most 4 accumulator tiles, the 64x64 block) load
+  // their old C values before the first operand tiles, so with beta != 0
+  // the two reads overlap instead of the epilogue waiting for a second
+  // round trip (the rank-32 and rank-128 updates of the blocked QR: one or
+  // a few k-tiles).
+  constexpr bool PRE_C = TM * TN <= 4;
+  const bool pre_c = PRE_C && g.beta != 0.0 && !g.partial;
+  double cpre[PRE_C ? TM : 1][PRE_C ? TN : 1][4];
+  if constexpr (PRE_C) {
+    if (pre_c) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cpre[i][j][v] = load_c(i, j, v);
+    }
+  }
   load_tiles(0);
   store_tiles(0);
   __syncthreads();
-
-  const int lr = lane & 15;   // row (A) / column (B) inside a 16x16 tile
-  const int lk = lane >> 4;   // k group of the lane
 
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
@@ -432,9 +460,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
     __syncthreads();
   }
 
-  // Epilogue. 16x16 accumulator tile, lane group lk = lane/16, item v:
-  //   f32 MFMAs: row = 4*lk + v ;  f64 MFMA: row = lk + 4*v ; column = lane%16
-  auto row_of = [&](int v) { return (PREC == 0) ? (lk + 4 * v) : (lk * 4 + v); };
+  // Epilogue. 16x16 accumulator tile, lane group lk = lane/16, item v
+  // (rows as in row_of above).
   if (g.partial) {
     double* P = g.partial + ((size_t)blockIdx.y * gridDim.z + split) * (size_t)g.M * g.N;
 #pragma unroll
@@ -449,11 +476,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
         }
     return;
   }
-  char* Cb = (char*)g.C + ((size_t)b1 * g.sC + (size_t)b2 * g.sC2) * (g.tc == 0 ? 8 : 4);
   // One 16-row band of the wave's tile at a time: with beta != 0 all of the
-  // band's old C values are loaded first, then combined and stored (a load
-  // placed after the previous element's store would wait for it, since the
-  // compiler cannot tell the addresses apart).
+  // band's old C values are loaded first (or were, before the k loop), then
+  // combined and stored (a load placed after the previous element's store
+  // would wait for it, since the compiler cannot tell the addresses apart).
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
     double cv[TN][4];
@@ -461,13 +487,12 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs g) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int m = m0 + wm0 + i * 16 + row_of(v);
-        const int n = n0 + wn0 + j * 16 + lr;
         cv[j][v] = 0.0;
-        if (g.beta != 0.0 && m < g.M && n < g.N) {
-          const int64_t off = (int64_t)m * g.ldc + n;
-          cv[j][v] = g.tc == 0 ? reinterpret_cast<const double*>(Cb)[off]
-                               : (double)reinterpret_cast<const float*>(Cb)[off];
+        if constexpr (PRE_C) {
+          if (pre_c) cv[j][v] = cpre[i][j][v];
+          else if (g.beta != 0.0) cv[j][v] = load_c(i, j, v);
+        } else {
+          if (g.beta != 0.0) cv[j][v] = load_c(i, j, v);
         }
       }
 #pragma unroll
