@@ -947,6 +947,17 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   }
   cq_gram_partial(Yr, q.Gp);
   if (!cq_arrive_last(q.cnt)) return;
+  // the reconstruction's inputs (L1 of the first pass, the top b rows of
+  // Q1), loaded now so that they arrive during the reduction and Cholesky
+  double pre_l1[kCqG / 256], pre_q1[kCqG / 256];
+  if (PASS > 0) {
+#pragma unroll
+    for (int u = 0; u < kCqG / 256; ++u) {
+      const int e = tid + 256 * u, i = e >> 5, c = e & 31;
+      pre_l1[u] = q.L1s[e];
+      pre_q1[u] = i < b && c < b ? q.Qb[(size_t)i * kTsNB + c] : 0.0;
+    }
+  }
   if (tid == 0) okf = 1;
   cq_gram_reduce(q.Gp, nblk, b, G);
   if (PASS == 0) {
@@ -986,11 +997,13 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   __shared__ double Lf[kTsNB * kCqLd];
   __shared__ double sg[kTsNB];
   __shared__ double rdU[kTsNB];
-  cq_load(q.L1s, Ma);
-  // top b rows of Q1 (written by the first workgroup above) -> X rows 0..31
-  for (int e = tid; e < kCqG; e += 256) {
-    const int i = e >> 5, c = e & 31;
-    X[i * kCqLd + c] = i < b && c < b ? q.Qb[(size_t)i * kTsNB + c] : 0.0;
+  // L1 -> Ma; the top b rows of Q1 (written by the first workgroup) -> X
+  // rows 0..31
+#pragma unroll
+  for (int u = 0; u < kCqG / 256; ++u) {
+    const int e = tid + 256 * u, i = e >> 5, c = e & 31;
+    Ma[i * kCqLd + c] = pre_l1[u];
+    X[i * kCqLd + c] = pre_q1[u];
   }
   __syncthreads();
   {  // R[i][c] = (L1 L2)[c][i] -> Mb (upper): one 16 x 16 tile per wave
@@ -1055,6 +1068,7 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
   __shared__ double Lf[kTsNB * kCqLd];
   __shared__ double Rs[kTsNB * kCqLd];
   __shared__ double Vt[kCqRows * kCqLd];
+  __shared__ double Pr[kCqRows * kCqLd];
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (*flag) return;
@@ -1065,9 +1079,12 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
   cq_load(q.T, T);
   cq_load(q.L1f, Lf);
   cq_load(q.Rs, Rs);
+  // the rows of Q1 and, for the check, of P, loaded together
   for (int e = tid; e < kCqRows * kTsNB; e += 256) {
     const int i = e >> 5, c = e & 31;
-    X[i * kCqLd + c] = r0 + i < mp && c < b ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
+    const bool in = r0 + i < mp && c < b;
+    X[i * kCqLd + c] = in ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
+    Pr[i * kCqLd + c] = in ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
   }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
@@ -1083,11 +1100,6 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
     }
   }
   __syncthreads();
-  for (int e = tid; e < kCqRows * kTsNB; e += 256) {
-    const int i = e >> 5, c = e & 31;
-    X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
-  }
-  __syncthreads();
   if (wave < 2) {  // |P - Q R| against 2e-14 ||P||_F
     const int c0 = wave * 16;
     const double thr = 2e-14 * *q.pnorm;
@@ -1098,7 +1110,7 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = lk + 4 * v, c = c0 + lr;
-      if (c < b && r0 + r < mp && !(fabs(X[r * kCqLd + c] - d[v]) <= thr)) fail = true;
+      if (c < b && r0 + r < mp && !(fabs(Pr[r * kCqLd + c] - d[v]) <= thr)) fail = true;
     }
     if (fail) bad = 1;
   } else {  // V rows: L1f for the top b rows, Q Ui below
