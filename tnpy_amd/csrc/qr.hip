@@ -898,12 +898,38 @@ __device__ __forceinline__ void cq_gram_partial(const double* Y, double* __restr
   }
 }
 
+// 32 x 32 blocks between LDS (row stride kCqLd) and global memory, by the
+// 256 threads of the workgroup, four entries each. The loads of cq_load go
+// to registers first, so several blocks' loads can be in flight together
+// (a strided loop that stores each value as it arrives waits for every
+// load in turn).
+constexpr int kCqPer = kCqG / 256;
+
 __device__ __forceinline__ void cq_store(const double* S, double* __restrict__ g) {
-  for (int e = threadIdx.x; e < kCqG; e += 256) g[e] = S[(e >> 5) * kCqLd + (e & 31)];
+#pragma unroll
+  for (int u = 0; u < kCqPer; ++u) {
+    const int e = threadIdx.x + 256 * u;
+    g[e] = S[(e >> 5) * kCqLd + (e & 31)];
+  }
 }
 
-__device__ __forceinline__ void cq_load(const double* __restrict__ g, double* S) {
-  for (int e = threadIdx.x; e < kCqG; e += 256) S[(e >> 5) * kCqLd + (e & 31)] = g[e];
+struct CqRegs {
+  double v[kCqPer];
+};
+
+__device__ __forceinline__ CqRegs cq_fetch(const double* __restrict__ g) {
+  CqRegs r;
+#pragma unroll
+  for (int u = 0; u < kCqPer; ++u) r.v[u] = g[threadIdx.x + 256 * u];
+  return r;
+}
+
+__device__ __forceinline__ void cq_put(const CqRegs& r, double* S) {
+#pragma unroll
+  for (int u = 0; u < kCqPer; ++u) {
+    const int e = threadIdx.x + 256 * u;
+    S[(e >> 5) * kCqLd + (e & 31)] = r.v[u];
+  }
 }
 
 __global__ void cq_init_kernel(int* cnt) { *cnt = 0; }
@@ -929,10 +955,24 @@ __global__ __launch_bounds__(256) void cq_pass_kernel(const double* __restrict__
   const int tid = threadIdx.x;
   if (PASS > 0 && *flag) return;
   const int r0 = blockIdx.x * kCqRB;
-  if (PASS > 0) cq_load(q.Li, Li);
-  for (int e = tid; e < kCqRB * kTsNB; e += 256) {
-    const int i = e >> 5, c = e & 31;
-    X[i * kCqLd + c] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+  {
+    // the panel's rows (and in the second pass L1^-1), all loads in flight
+    // before the first LDS store
+    constexpr int UX = kCqRB * kTsNB / 256;
+    double xv[UX];
+#pragma unroll
+    for (int u = 0; u < UX; ++u) {
+      const int e = tid + 256 * u, i = e >> 5, c = e & 31;
+      xv[u] = r0 + i < mp && c < b ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+    }
+    CqRegs li;
+    if (PASS > 0) li = cq_fetch(q.Li);
+#pragma unroll
+    for (int u = 0; u < UX; ++u) {
+      const int e = tid + 256 * u;
+      X[(e >> 5) * kCqLd + (e & 31)] = xv[u];
+    }
+    if (PASS > 0) cq_put(li, Li);
   }
   __syncthreads();
   const double* Yr = X;
@@ -1074,17 +1114,31 @@ __global__ __launch_bounds__(256) void cq_rows_kernel(double* __restrict__ A, in
   if (*flag) return;
   const int r0 = blockIdx.x * kCqRows;
   if (tid == 0) bad = 0;
-  cq_load(q.Li, Li);
-  cq_load(q.Ui, Ui);
-  cq_load(q.T, T);
-  cq_load(q.L1f, Lf);
-  cq_load(q.Rs, Rs);
-  // the rows of Q1 and, for the check, of P, loaded together
-  for (int e = tid; e < kCqRows * kTsNB; e += 256) {
-    const int i = e >> 5, c = e & 31;
-    const bool in = r0 + i < mp && c < b;
-    X[i * kCqLd + c] = in ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
-    Pr[i * kCqLd + c] = in ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+  {
+    // five 32 x 32 factors, the rows of Q1 and, for the check, of P: all
+    // loads in flight before the first LDS store
+    const CqRegs li = cq_fetch(q.Li), ui = cq_fetch(q.Ui), tt = cq_fetch(q.T),
+                 lf = cq_fetch(q.L1f), rs = cq_fetch(q.Rs);
+    constexpr int UR = kCqRows * kTsNB / 256;
+    double xv[UR], pv[UR];
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const int e = tid + 256 * u, i = e >> 5, c = e & 31;
+      const bool in = r0 + i < mp && c < b;
+      xv[u] = in ? q.Qb[(size_t)(r0 + i) * kTsNB + c] : 0.0;
+      pv[u] = in ? A[(int64_t)(jb + r0 + i) * n + jb + c] : 0.0;
+    }
+    cq_put(li, Li);
+    cq_put(ui, Ui);
+    cq_put(tt, T);
+    cq_put(lf, Lf);
+    cq_put(rs, Rs);
+#pragma unroll
+    for (int u = 0; u < UR; ++u) {
+      const int e = tid + 256 * u;
+      X[(e >> 5) * kCqLd + (e & 31)] = xv[u];
+      Pr[(e >> 5) * kCqLd + (e & 31)] = pv[u];
+    }
   }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
