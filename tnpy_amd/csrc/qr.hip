@@ -899,10 +899,10 @@ __device__ __forceinline__ void cq_gram_partial(const double* Y, double* __restr
 }
 
 // 32 x 32 blocks between LDS (row stride kCqLd) and global memory, by the
-// 256 threads of the workgroup, four entries each. The loads of cq_load go
-// to registers first, so several blocks' loads can be in flight together
-// (a strided loop that stores each value as it arrives waits for every
-// load in turn).
+// 256 threads of the workgroup, four entries each. cq_fetch loads a block
+// into registers and cq_put stores it to LDS, so the loads of several
+// blocks can be in flight together (a strided loop that stores each value
+// as it arrives waits for every load in turn).
 constexpr int kCqPer = kCqG / 256;
 
 __device__ __forceinline__ void cq_store(const double* S, double* __restrict__ g) {
